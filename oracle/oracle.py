@@ -1,0 +1,338 @@
+"""CPU oracle for the two-electron Fock build - TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this module; the product (``quick_amd``) never does.
+
+* :class:`QuickBasisRef` restates QUICK's basis reader and normalisation
+  literally (``src/basis.f90:370-720``, ``src/subs/xnorm.f90``), including the
+  way the d-shell coefficient used by the integrals is taken from the yy
+  function (``eri.fh:131-140`` indexes ``gccoeff(:, ksumtype+L)``).
+* :class:`Oracle` wraps ``liboracle_quick.so`` (``quick_oracle.c``): FmT,
+  Schwarz screening, getCshellEri, 1e integrals.
+* :func:`rhf` is a plain DIIS RHF driver (``quick_scf_module.f90:344-760``
+  restated: Pulay DIIS on FPS-SPF in the orthogonal basis, 10 vectors,
+  ``P = 2 C_occ C_occ^T``); it only needs to reach the same converged state
+  as QUICK, so the guess is the core Hamiltonian.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle_quick.so")
+BASIS_DIR = os.path.join(os.path.dirname(HERE), "quick_amd", "basis_data")
+PI = 3.14159265358979323846264338327950288
+BOHR = 0.52917720859
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"oracle library missing: {LIBPATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIBPATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.qo_create.restype = ctypes.c_void_p
+        L.qo_create.argtypes = [ctypes.c_void_p]
+        L.qo_destroy.argtypes = [ctypes.c_void_p]
+        L.qo_get_schwarz.argtypes = [ctypes.c_void_p, dp, dp]
+        L.qo_density_cutoff.argtypes = [ctypes.c_void_p, dp, dp]
+        L.qo_get2e.restype = ctypes.c_long
+        L.qo_get2e.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, dp]
+        L.qo_get2e_omp.restype = ctypes.c_long
+        L.qo_get2e_omp.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, dp]
+        L.qo_get2e_range.restype = ctypes.c_long
+        L.qo_get2e_range.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
+        L.qo_fmt.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+        L.qo_one_electron.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.qo_eri_tensor.argtypes = [ctypes.c_void_p, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def fmt(maxm, x):
+    out = np.zeros(maxm + 1 + 4)
+    lib().qo_fmt(int(maxm), float(x), _p(out))
+    return out[:maxm + 1].copy()
+
+
+# ---------------------------------------------------------------------------
+# Basis (restating QUICK's reader)
+# ---------------------------------------------------------------------------
+ELEMENTS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne",
+            "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar"]
+_FILES = {"sto-3g": "STO-3G", "3-21g": "3-21G", "6-31g": "6-31G", "6-31g*": "6-31Gs",
+          "6-31g(d)": "6-31Gs", "6-31g**": "6-31Gss", "6-31g(d,p)": "6-31Gss",
+          "cc-pvdz": "cc-pVDZ", "def2-svp": "def2-SVP"}
+
+
+def xnorm(a, i, j, k):
+    """xnorm.f90:22-61"""
+    x = (2.0 * a) ** ((i + j + k) / 2.0 + 0.75)
+    g = 1.0
+    for L in range(1, i + 1):
+        g *= (i - L) + 0.5
+    for L in range(1, j + 1):
+        g *= (j - L) + 0.5
+    for L in range(1, k + 1):
+        g *= (k - L) + 0.5
+    return x * PI ** (-0.75) * g ** (-0.5)
+
+
+def xnewnorm(i, j, k, coef, expo):
+    """xnorm.f90:64-97"""
+    xponent = -1.5 - (i + j + k)
+    s = 0.0
+    for c1, a1 in zip(coef, expo):
+        for c2, a2 in zip(coef, expo):
+            s += c1 * c2 * (a1 + a2) ** xponent
+    g = 1.0
+    for L in range(1, i + 1):
+        g *= (i - L) + 0.5
+    for L in range(1, j + 1):
+        g *= (j - L) + 0.5
+    for L in range(1, k + 1):
+        g *= (k - L) + 0.5
+    return (s * g * PI ** 1.5) ** (-0.5)
+
+
+class QuickBasisRef:
+    """QUICK's shell arrays for a molecule (basis.f90:370-720)."""
+
+    def __init__(self, symbols, xyz_bohr, basis_name, charges=None):
+        with open(os.path.join(BASIS_DIR, _FILES[basis_name.lower()] + ".json")) as fh:
+            doc = json.load(fh)
+        self.symbols = list(symbols)
+        self.xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        self.charge = np.array([float(ELEMENTS.index(s)) for s in symbols]) if charges is None \
+            else np.asarray(charges, dtype=np.float64)
+        katom, qstart, qfinal, kprim, kstart, ksumtype = [], [], [], [], [], []
+        gcexpo, gccoeff_fn = [], []   # per function: list of per-prim coefficients
+        fn_first = 0
+        jb = 0
+        per_shell_coef = []
+        for ia, s in enumerate(symbols):
+            for sh in doc["elements"][s]:
+                t = sh["type"]
+                AA = sh["exps"]
+                n = len(AA)
+                katom.append(ia)
+                kprim.append(n)
+                kstart.append(jb)
+                ksumtype.append(fn_first)
+                gcexpo.extend(AA)
+                jb += n
+                funcs = []
+                if t == "S":
+                    qstart.append(0); qfinal.append(0)
+                    c = [b * xnorm(a, 0, 0, 0) for a, b in zip(AA, sh["coefs"][0])]
+                    w = xnewnorm(0, 0, 0, c, AA)
+                    funcs.append([w * x for x in c])
+                elif t == "P":
+                    qstart.append(1); qfinal.append(1)
+                    c = [b * xnorm(a, 1, 0, 0) for a, b in zip(AA, sh["coefs"][0])]
+                    w = xnewnorm(1, 0, 0, c, AA)
+                    funcs += [[w * x for x in c]] * 3
+                elif t == "SP":
+                    qstart.append(0); qfinal.append(1)
+                    c = [b * xnorm(a, 0, 0, 0) for a, b in zip(AA, sh["coefs"][0])]
+                    w = xnewnorm(0, 0, 0, c, AA)
+                    funcs.append([w * x for x in c])
+                    c = [b * xnorm(a, 1, 0, 0) for a, b in zip(AA, sh["coefs"][1])]
+                    w = xnewnorm(1, 0, 0, c, AA)
+                    funcs += [[w * x for x in c]] * 3
+                elif t == "D":
+                    qstart.append(2); qfinal.append(2)
+                    klmn = [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)]
+                    raw = [[b * xnorm(a, *k) for a, b in zip(AA, sh["coefs"][0])] for k in klmn]
+                    w1 = xnewnorm(2, 0, 0, raw[2], AA)     # from the yy function
+                    w2 = xnewnorm(1, 1, 0, raw[5], AA)     # from the zz function
+                    funcs = [[w1 * x for x in raw[k]] for k in range(3)] + \
+                            [[w2 * x for x in raw[k]] for k in range(3, 6)]
+                else:
+                    raise ValueError("shell type " + t)
+                per_shell_coef.append(funcs)
+                fn_first += len(funcs)
+        self.nbasis = fn_first
+        self.nshell = len(katom)
+        self.jbasis = jb
+        self.katom = np.array(katom, dtype=np.int32)
+        self.qstart = np.array(qstart, dtype=np.int32)
+        self.qfinal = np.array(qfinal, dtype=np.int32)
+        self.kprim = np.array(kprim, dtype=np.int32)
+        self.kstart = np.array(kstart, dtype=np.int32)
+        self.ksumtype = np.array(ksumtype, dtype=np.int32)
+        self.gcexpo = np.array(gcexpo, dtype=np.float64)
+        gcc = np.zeros((jb, 4))
+        for s in range(self.nshell):
+            funcs = per_shell_coef[s]
+            for L in range(qstart[s], qfinal[s] + 1):
+                # QUICK takes gccoeff of function ksumtype + L (Xcoeff, eri.fh:131-140)
+                gcc[kstart[s]:kstart[s] + kprim[s], L] = funcs[L]
+        self.gccoeff = gcc
+
+
+class _QoBasis(ctypes.Structure):
+    _fields_ = [("natom", ctypes.c_int), ("xyz", ctypes.c_void_p), ("charge", ctypes.c_void_p),
+                ("nshell", ctypes.c_int), ("katom", ctypes.c_void_p), ("qstart", ctypes.c_void_p),
+                ("qfinal", ctypes.c_void_p), ("kprim", ctypes.c_void_p), ("kstart", ctypes.c_void_p),
+                ("ksumtype", ctypes.c_void_p), ("nbasis", ctypes.c_int), ("jbasis", ctypes.c_int),
+                ("gcexpo", ctypes.c_void_p), ("gccoeff", ctypes.c_void_p)]
+
+
+class Oracle:
+    """QUICK's CPU getCshellEri restated (quick_oracle.c)."""
+
+    def __init__(self, qb: QuickBasisRef):
+        self.qb = qb
+        self._keep = [qb.xyz, qb.charge, qb.katom, qb.qstart, qb.qfinal, qb.kprim, qb.kstart,
+                      qb.ksumtype, qb.gcexpo, np.ascontiguousarray(qb.gccoeff)]
+        s = _QoBasis(len(qb.symbols), qb.xyz.ctypes.data, qb.charge.ctypes.data, qb.nshell,
+                     qb.katom.ctypes.data, qb.qstart.ctypes.data, qb.qfinal.ctypes.data,
+                     qb.kprim.ctypes.data, qb.kstart.ctypes.data, qb.ksumtype.ctypes.data,
+                     qb.nbasis, qb.jbasis, qb.gcexpo.ctypes.data, self._keep[-1].ctypes.data)
+        self.h = lib().qo_create(ctypes.byref(s))
+        self.n = qb.nbasis
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().qo_destroy(self.h)
+            self.h = None
+
+    def schwarz(self):
+        y = np.zeros((self.qb.nshell, self.qb.nshell))
+        c = np.zeros((self.qb.jbasis, self.qb.jbasis))
+        lib().qo_get_schwarz(self.h, _p(y), _p(c))
+        return y, c
+
+    def density_cutoff(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        cm = np.zeros((self.qb.nshell, self.qb.nshell))
+        lib().qo_density_cutoff(self.h, _p(P), _p(cm))
+        return cm
+
+    def fock2e(self, P, cutoff=1e-12, prim_limit=None, hyb=1.0, threads=1, cutmatrix=None):
+        """G(P) added to zero, symmetric; returns (G, number of shell quartets)."""
+        if prim_limit is None:
+            prim_limit = 0.1 * cutoff
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        o = np.zeros((self.n, self.n))
+        cm = None if cutmatrix is None else np.ascontiguousarray(cutmatrix, dtype=np.float64)
+        cmp = _p(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
+        if threads == 1:
+            nq = lib().qo_get2e(self.h, _p(P), cmp, cutoff, prim_limit, hyb, _p(o))
+        else:
+            nq = lib().qo_get2e_omp(self.h, _p(P), cmp, cutoff, prim_limit, hyb, threads, _p(o))
+        return o, nq
+
+    def fock2e_shell_range(self, P, ii_begin, ii_end, cutoff=1e-12, prim_limit=None, hyb=1.0):
+        if prim_limit is None:
+            prim_limit = 0.1 * cutoff
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        o = np.zeros((self.n, self.n))
+        nq = lib().qo_get2e_range(self.h, _p(P), ctypes.POINTER(ctypes.c_double)(), cutoff,
+                                  prim_limit, hyb, ii_begin, ii_end, _p(o))
+        return o, nq
+
+    def one_electron(self):
+        n = self.n
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        lib().qo_one_electron(self.h, _p(S), _p(T), _p(V))
+        return S, T, V
+
+    def eri_tensor(self):
+        n = self.n
+        e = np.zeros((n, n, n, n))
+        lib().qo_eri_tensor(self.h, _p(e))
+        return e
+
+
+def nuclear_repulsion(charges, xyz):
+    e = 0.0
+    for i in range(len(charges)):
+        for j in range(i + 1, len(charges)):
+            e += charges[i] * charges[j] / np.linalg.norm(xyz[i] - xyz[j])
+    return e
+
+
+def seeded_density(S, nocc, seed=20261202):
+    """P = 2 C C^T with C (n x nocc) ~ N(0,1) from default_rng(seed), then
+    S-orthonormalised C <- C (C^T S C)^(-1/2): rank nocc, tr(PS) = 2 nocc."""
+    n = S.shape[0]
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, nocc))
+    M = C.T @ S @ C
+    w, U = np.linalg.eigh(M)
+    C = C @ (U @ np.diag(w ** -0.5) @ U.T)
+    return 2.0 * C @ C.T
+
+
+def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=1e-10,
+        dtol=1e-8, ndiis=10, verbose=False):
+    """Plain-DIIS RHF with full (non-incremental) Fock builds by the oracle."""
+    qb = oracle.qb
+    S, T, V = oracle.one_electron()
+    H = T + V
+    s, U = np.linalg.eigh(S)
+    X = U @ np.diag(s ** -0.5) @ U.T
+    nocc = nelec // 2
+    enuc = nuclear_repulsion(qb.charge, qb.xyz)
+
+    def density(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        return 2.0 * C[:, :nocc] @ C[:, :nocc].T
+
+    P = density(H)
+    E_old = 0.0
+    fs, es = [], []
+    hist = []
+    for it in range(maxiter):
+        G, _ = oracle.fock2e(P, cutoff=cutoff, prim_limit=prim_limit)
+        F = H + G
+        E = 0.5 * np.sum(P * (H + F)) + enuc
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        fs.append(F)
+        es.append(err)
+        fs, es = fs[-ndiis:], es[-ndiis:]
+        k = len(fs)
+        B = -np.ones((k + 1, k + 1))
+        B[k, k] = 0.0
+        for i in range(k):
+            for j in range(k):
+                B[i, j] = np.sum(es[i] * es[j])
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            coef = np.linalg.solve(B, rhs)[:k]
+            Fd = sum(c * f for c, f in zip(coef, fs))
+        except np.linalg.LinAlgError:
+            Fd = F
+        Pn = density(Fd)
+        dP = np.sqrt(np.mean((Pn - P) ** 2))
+        hist.append(E)
+        if verbose:
+            print(f"{it:3d} {E:.12f} {E - E_old:.3e} {dP:.3e}")
+        if abs(E - E_old) < etol and dP < dtol:
+            return E, P, hist
+        E_old = E
+        P = Pn
+    raise RuntimeError("SCF did not converge")

@@ -1,0 +1,88 @@
+"""Collect QUICK's own RHF regression fixtures into a small JSON file.
+
+Reads the reference's ``test/ene_*_rhf_*.in`` inputs (geometry, basis,
+charge) and the matching ``test/saved/*.out`` outputs (TOTAL ENERGY,
+ELECTRONIC ENERGY, CORE_CORE REPULSION, and the SCF cycle count) and writes
+``saved_rhf_energies.json`` beside this script.  Only data is kept: inputs and
+expected outputs.
+
+    python tests/golden/make_saved_energies.py /root/reference/test
+"""
+import glob
+import json
+import os
+import re
+import sys
+
+BASES = {"STO-3G": "STO-3G", "3-21G": "3-21G", "6-31G": "6-31G", "6-31G*": "6-31G*",
+         "6-31G**": "6-31G**", "cc-pvdz": "cc-pVDZ"}
+KEEP = {"H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si",
+        "P", "S", "Cl", "Ar"}
+
+
+def parse_in(path):
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    kw = lines[0]
+    m = re.search(r"BASIS=(\S+)", kw)
+    basis = m.group(1)
+    m = re.search(r"CHARGE=([+-]?\d+)", kw)
+    charge = int(m.group(1)) if m else 0
+    m = re.search(r"cutoff=(\S+)", kw, re.I)
+    cutoff = float(m.group(1)) if m else 1e-7
+    syms, xyz = [], []
+    for ln in lines[2:]:
+        t = ln.split()
+        if len(t) != 4:
+            if syms:
+                break
+            continue
+        syms.append(t[0].capitalize())
+        xyz.append([float(v) for v in t[1:]])
+    return basis, charge, cutoff, syms, xyz, kw
+
+
+def parse_out(path):
+    out = {}
+    with open(path) as fh:
+        txt = fh.read()
+    for key, tag in [("E_total", "TOTAL ENERGY"), ("E_elec", "ELECTRONIC ENERGY"),
+                     ("E_nuc", "CORE_CORE REPULSION")]:
+        m = re.search(tag + r"\s*=\s*(-?\d+\.\d+)", txt)
+        if m:
+            out[key] = float(m.group(1))
+    m = re.search(r"REACH CONVERGENCE AFTER\s+(\d+)", txt)
+    if m:
+        out["cycles"] = int(m.group(1))
+    m = re.findall(r"BASIS FUNCTIONS =\s+(\d+)", txt)
+    if m:
+        out["nbasis"] = int(m[-1])   # the last one is the molecule (earlier ones: SAD atoms)
+    return out
+
+
+def main(test_dir):
+    recs = []
+    for inp in sorted(glob.glob(os.path.join(test_dir, "ene_*_rhf_*.in"))):
+        name = os.path.basename(inp)[:-3]
+        outp = os.path.join(test_dir, "saved", name + ".out")
+        if not os.path.exists(outp):
+            continue
+        basis, charge, cutoff, syms, xyz, kw = parse_in(inp)
+        if basis not in BASES or not set(syms) <= KEEP:
+            continue
+        if "DFT" in kw.upper() or "B3LYP" in kw.upper():
+            continue
+        res = parse_out(outp)
+        if "E_total" not in res:
+            continue
+        recs.append(dict(name=name, basis=BASES[basis], charge=charge, cutoff=cutoff,
+                         symbols=syms, xyz_angstrom=xyz, **res))
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "saved_rhf_energies.json")
+    with open(dst, "w") as fh:
+        json.dump({"source": "QUICK test/saved regression outputs (printed to 1e-9 Eh)",
+                   "records": recs}, fh, indent=1)
+    print(len(recs), "records ->", dst)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/test")
