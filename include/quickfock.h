@@ -1,0 +1,167 @@
+/*
+ * quickfock.h - C ABI of libquickfock, the MI355X (gfx950) two-electron
+ * Fock-build engine that replaces QUICK's gpu_get2e path.
+ *
+ * Two layers:
+ *
+ *  1. QUICK-compatible entry points (lower case, trailing underscore, every
+ *     argument by reference, Fortran column-major arrays).  They keep QUICK's
+ *     names, argument order and meaning so QUICK's Fortran scf_operator can
+ *     call them unchanged.  Each cites the QUICK function it replaces.
+ *
+ *  2. The native API (qf_*): an explicit engine handle, device or host
+ *     buffers, a HIP stream, precision mode, batched builds over many
+ *     molecules, and error codes.  The QUICK layer is a thin shim over it.
+ *
+ * All matrices are nbasis x nbasis FP64.  Density and Fock matrices are
+ * symmetric, so row- and column-major layouts coincide.
+ */
+#ifndef QUICKFOCK_H
+#define QUICKFOCK_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Error codes                                                         */
+/* ------------------------------------------------------------------ */
+enum {
+    QF_OK = 0,
+    QF_ERR_ARG = 1,        /* bad argument / unsupported angular momentum */
+    QF_ERR_STATE = 2,      /* call out of order (e.g. build before prepare) */
+    QF_ERR_HIP = 3,        /* a HIP runtime call failed */
+    QF_ERR_NODEVICE = 24,  /* QUICK's ierr 24: no GPU */
+    QF_ERR_BADDEVICE = 25, /* QUICK's ierr 25: bad device id */
+    QF_ERR_MEMORY = 30     /* device memory budget exceeded */
+};
+
+enum { QF_PREC_FP64 = 0, QF_PREC_MIXED = 1 };
+
+typedef struct qf_engine qf_engine;
+
+typedef struct {
+    long long n_pairs;           /* shell pairs kept after Schwarz */
+    long long n_tasks_fp64;      /* wave tasks (1 bra x <=64 kets) */
+    long long n_tasks_fp32;
+    long long n_quartets_bound;  /* quartets passing the Schwarz-only test */
+    long long device_bytes;      /* device memory held by the engine */
+} qf_stats;
+
+/* Last error message of the calling thread (never NULL). */
+const char *qf_last_error(void);
+
+/* Engine lifetime.  device < 0 selects the current device. */
+int qf_engine_create(int device, qf_engine **out);
+void qf_engine_destroy(qf_engine *e);
+
+/* Geometry in bohr, xyz[3*natom]. */
+int qf_set_molecule(qf_engine *e, int natom, const double *xyz);
+
+/* Basis in QUICK's shell layout (0-based indices):
+ *   shell_atom[s], shell_lmin[s] (Qstart), shell_lmax[s] (Qfinal; SP = 0..1),
+ *   shell_nprim[s] (kprim), shell_pstart[s] (kstart), shell_fstart[s]
+ *   (ksumtype: first basis function), exps[nprim_total] (gcexpo) and
+ *   coef[nprim_total*4] the normalised coefficient of primitive p for L
+ *   (QUICK's gccoeff of function ksumtype+L), at coef[p*4+L].
+ *   Angular momentum must be <= 2. */
+int qf_set_basis(qf_engine *e, int nshell, const int *shell_atom, const int *shell_lmin,
+                 const int *shell_lmax, const int *shell_nprim, const int *shell_pstart,
+                 const int *shell_fstart, int nprim_total, const double *exps, const double *coef,
+                 int nbasis);
+
+/* Screening thresholds: integral cutoff, primitive limit (QUICK's
+ * integralCutoff, primLimit) and exchange weight (x_hybrid_coeff, 1 = HF). */
+int qf_set_cutoffs(qf_engine *e, double integral_cutoff, double prim_limit, double hyb);
+
+/* Precision: QF_PREC_FP64, or QF_PREC_MIXED where shell quartets with
+ * Ycut_ij*Ycut_kl < split*max(Ycut_ij*Ycut_kl) run VRR/HRR in FP32
+ * (Boys, bounds and Fock accumulation stay FP64). */
+int qf_set_precision(qf_engine *e, int mode, double split);
+
+/* Schwarz bounds.  Pass NULL to have them computed on the GPU, or QUICK's
+ * Ycutoff[nshell*nshell] and cutprim[nprim_total*nprim_total]. */
+int qf_set_schwarz(qf_engine *e, const double *ycutoff, const double *cutprim);
+int qf_get_schwarz(qf_engine *e, double *ycutoff, double *cutprim);
+
+/* Build pair lists, bounds and wave task lists (after the calls above). */
+int qf_prepare(qf_engine *e);
+int qf_get_stats(qf_engine *e, qf_stats *out);
+
+/* F += G(P) for a device-resident density.  d_F may equal NULL, in which
+ * case G is left in the engine and can be read with qf_copy_g.  d_cutmat
+ * (nshell*nshell max |P| per shell pair) may be NULL: computed on device.
+ * stream is a hipStream_t (NULL = default stream).  Asynchronous. */
+int qf_fock_build_device(qf_engine *e, const double *d_P, double *d_F, const double *d_cutmat,
+                         void *stream);
+
+/* Host convenience: F[n*n] += G(P[n*n]); synchronous. */
+int qf_fock_build_host(qf_engine *e, const double *P, double *F, const double *cutmat);
+
+/* Device pointer of the engine's G (= Gt + Gt^T after the last build). */
+const double *qf_device_g(qf_engine *e);
+
+/* Batched build: one launch per class covers every engine in the list.
+ * d_P[i] / d_F[i] are device pointers per engine.  All engines must be
+ * prepared and live on the same device. */
+int qf_fock_build_batch(qf_engine *const *engines, int count, const double *const *d_P,
+                        double *const *d_F, void *stream);
+
+/* One-electron matrices for the SCF harness: S, T, V (nbasis*nbasis, host)
+ * with nuclear charges[natom]; computed on the GPU. */
+int qf_one_electron(qf_engine *e, const double *charges, double *S, double *T, double *V);
+
+/* Algorithmic FLOPs of the last build (per the committed class FLOP table,
+ * counted on the host for the Schwarz-surviving quartets). */
+double qf_flops_estimate(qf_engine *e);
+
+/* ------------------------------------------------------------------ */
+/* QUICK-compatible layer (src/gpu/hip/gpu.cu, src/gpu/gpu_get2e_getxc_drivers.h) */
+/* ------------------------------------------------------------------ */
+/* gpu.cu:197 gpu_new_ */
+void gpu_new_(int *ierr);
+/* gpu.cu:184 gpu_set_device_ */
+void gpu_set_device_(int *gpu_dev_id, int *ierr);
+/* gpu.cu:281 gpu_init_device_ */
+void gpu_init_device_(int *ierr);
+/* gpu.cu:437 gpu_allocate_scratch_ (nothing to allocate: kernels keep no global scratch) */
+void gpu_allocate_scratch_(bool *allocate_gradient_scratch);
+/* gpu.cu:496 gpu_deallocate_scratch_ */
+void gpu_deallocate_scratch_(bool *deallocate_gradient_scratch);
+/* gpu.cu:514 gpu_delete_ */
+void gpu_delete_(int *ierr);
+/* gpu.cu:722 gpu_upload_method_ (0 = HF, hyb forced to 1) */
+void gpu_upload_method_(int *quick_method, bool *is_oshell, double *hyb_coeff);
+/* gpu.cu:539 gpu_setup_ */
+void gpu_setup_(int *natom, int *nbasis, int *nElec, int *imult, int *molchg, int *iAtomType);
+/* gpu.cu:776 gpu_upload_xyz_ (xyz(3, natom), bohr) */
+void gpu_upload_xyz_(double *atom_xyz);
+/* gpu.cu:820 gpu_upload_atom_and_chg_ */
+void gpu_upload_atom_and_chg_(int *atom, double *atom_chg);
+/* gpu.cu:1782 gpu_upload_basis_ (Fortran 1-based indices; gccoeff/gcexpo (MAXPRIM=14, nbasis)) */
+void gpu_upload_basis_(int *nshell, int *nprim, int *jshell, int *jbasis, int *maxcontract,
+                       int *ncontract, int *itype, double *aexp, double *dcoeff,
+                       int *first_basis_function, int *last_basis_function,
+                       int *first_shell_basis_function, int *last_shell_basis_function, int *ncenter,
+                       int *kstart, int *katom, int *ktype, int *kprim, int *kshell, int *Ksumtype,
+                       int *Qnumber, int *Qstart, int *Qfinal, int *Qsbasis, int *Qfbasis,
+                       double *gccoeff, double *cons, double *gcexpo, int *KLMN);
+/* gpu.cu:879 gpu_upload_cutoff_matrix_ (YCutoff(nshell,nshell), cutPrim(jbasis,jbasis)) */
+void gpu_upload_cutoff_matrix_(double *YCutoff, double *cutPrim);
+/* gpu.cu:1690 gpu_upload_calculated_ (only dense is read) */
+void gpu_upload_calculated_(double *o, double *co, double *vec, double *dense);
+/* gpu.cu:840 gpu_upload_cutoff_ */
+void gpu_upload_cutoff_(double *cutMatrix, double *integralCutoff, double *primLimit, double *DMCutoff,
+                        double *coreIntegralCutoff);
+/* gpu_get2e_getxc_drivers.h:27 gpu_get_cshell_eri_: o += G(dense), both triangles */
+void gpu_get_cshell_eri_(bool *deltaO, double *o);
+/* gpu.cu:2882 gpu_cleanup_ */
+void gpu_cleanup_(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

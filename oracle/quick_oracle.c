@@ -911,8 +911,11 @@ void qo_one_electron(const qo_ctx *c, double *Smat, double *Tmat, double *Vmat)
                     double kin = s0 * (Td[0] * Sy * Sz + Sx * Td[1] * Sz + Sx * Sy * Td[2]);
                     s_sum += ca * cb * ovl;
                     t_sum += ca * cb * kin;
-                    /* nuclear attraction: OS recursion on a then b with auxiliary m */
+                    /* nuclear attraction: OS recursion on a then b with auxiliary m;
+                     * FmT is called with Maxm = Qfinal(I)+Qfinal(J) as attrashell
+                     * (attrashell.fh:46-48,89) */
                     const int L = La + Lb;
+                    const int maxm = c->qfinal[sa] + c->qfinal[sb];
                     double vn = 0.0;
                     for (int at = 0; at < c->natom; ++at) {
                         const double *Cn = c->xyz + 3 * at;
@@ -922,7 +925,7 @@ void qo_one_electron(const qo_ctx *c, double *Smat, double *Tmat, double *Vmat)
                             PC2 += PC[d] * PC[d];
                         }
                         double fm[16];
-                        qo_fmt(L, p * PC2, fm);
+                        qo_fmt(maxm, p * PC2, fm);
                         /* table over (a-comp, b-comp, m): use lmn up to la/lb */
                         double V[3][3][3][3][3][3][5];
                         /* V[ax][ay][az][bx][by][bz][m]; fill base */

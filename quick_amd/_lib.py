@@ -1,0 +1,78 @@
+"""ctypes binding of libquickfock.so (the C ABI declared in include/quickfock.h).
+
+The library is built in-tree by ``python -m quick_amd.build``; importing this
+module raises if it is missing - there is no CPU fallback for the Fock build.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libquickfock.so")
+
+_lib = None
+
+QF_OK = 0
+QF_PREC_FP64 = 0
+QF_PREC_MIXED = 1
+
+
+class QfStats(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_longlong), ("n_tasks_fp64", ctypes.c_longlong),
+                ("n_tasks_fp32", ctypes.c_longlong), ("n_quartets_bound", ctypes.c_longlong),
+                ("device_bytes", ctypes.c_longlong)]
+
+
+class QuickFockError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise QuickFockError(f"{LIBPATH} not found: build it with `python -m quick_amd.build`")
+    L = ctypes.CDLL(LIBPATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "qf_last_error": (ctypes.c_char_p, []),
+        "qf_engine_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "qf_engine_destroy": (None, [vp]),
+        "qf_set_molecule": (ctypes.c_int, [vp, ctypes.c_int, dp]),
+        "qf_set_basis": (ctypes.c_int, [vp, ctypes.c_int, ip, ip, ip, ip, ip, ip, ctypes.c_int, dp, dp,
+                                        ctypes.c_int]),
+        "qf_set_cutoffs": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "qf_set_precision": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+        "qf_set_schwarz": (ctypes.c_int, [vp, dp, dp]),
+        "qf_get_schwarz": (ctypes.c_int, [vp, dp, dp]),
+        "qf_prepare": (ctypes.c_int, [vp]),
+        "qf_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(QfStats)]),
+        "qf_fock_build_device": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "qf_fock_build_host": (ctypes.c_int, [vp, dp, dp, dp]),
+        "qf_device_g": (vp, [vp]),
+        "qf_fock_build_batch": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp),
+                                               ctypes.POINTER(vp), vp]),
+        "qf_one_electron": (ctypes.c_int, [vp, dp, dp, dp, dp]),
+        "qf_flops_estimate": (ctypes.c_double, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != QF_OK:
+        msg = lib().qf_last_error().decode(errors="replace")
+        raise QuickFockError(f"{what} failed (code {rc}): {msg}")
+
+
+def dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))

@@ -1,0 +1,398 @@
+"""Generate the per-class ERI recursions for the HIP kernels.
+
+For every canonical class (La Lb | Lc Ld) with La >= Lb, Lc >= Ld, and bra
+pair class >= ket pair class (21 classes for s, p, d) this emits a struct
+with two straight-line device functions:
+
+* ``vrr`` - Obara-Saika / Head-Gordon-Pople vertical recursion for one
+  primitive quartet: builds [e0|f0]^(0) for every Cartesian e with
+  La <= |e| <= La+Lb and f with Lc <= |f| <= Lc+Ld from [00|00]^(m) and adds
+  them to the contracted accumulators (HGP 1988, eqs. 6 and 16 - the same
+  recurrences QUICK's ``vertical`` / ``ERint_vertical_spd`` evaluate).
+* ``hrr`` - horizontal recursion on the contracted accumulators:
+  ket (e0|c d+1_i) = (e0|c+1_i d) + CD_i (e0|c d), then bra
+  (a b+1_i|cd) = (a+1_i b|cd) + AB_i (a b|cd), producing the (ab|cd) block
+  in QUICK's component order (s; x y z; xx xy yy xz yz zz).
+
+The generator also counts the floating-point operations of each recursion
+(FMA = 2), which is the per-class FLOP table used for the roofline.
+
+    python -m quick_amd.codegen.gen_eri   # writes quick_amd/csrc/gen/*.h
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(os.path.dirname(HERE), "csrc", "gen")
+
+QCOMP = {
+    0: [(0, 0, 0)],
+    1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    2: [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)],
+}
+
+
+def comps(L):
+    if L in QCOMP:
+        return QCOMP[L]
+    out = []
+    for l in range(L, -1, -1):
+        for m in range(L - l, -1, -1):
+            out.append((l, m, L - l - m))
+    return out
+
+
+def am(t):
+    return t[0] + t[1] + t[2]
+
+
+def sub(t, i, k=1):
+    u = list(t)
+    u[i] -= k
+    return tuple(u)
+
+
+def add(t, i, k=1):
+    u = list(t)
+    u[i] += k
+    return tuple(u)
+
+
+def tag(t):
+    return "%d%d%d" % t
+
+
+Z = (0, 0, 0)
+
+
+class Vrr:
+    """Dependency graph of the vertical recursion for one class."""
+
+    def __init__(self, La, Lb, Lc, Ld):
+        self.La, self.Lb, self.Lc, self.Ld = La, Lb, Lc, Ld
+        self.L = La + Lb + Lc + Ld
+        self.nodes = {}   # (e, f, m) -> (kind, deps)
+        self.targets = []
+        for E in range(La, La + Lb + 1):
+            for e in comps(E):
+                for F in range(Lc, Lc + Ld + 1):
+                    for f in comps(F):
+                        self.targets.append((e, f))
+        for e, f in self.targets:
+            self.need(e, f, 0)
+
+    def choose_ket(self, e, f):
+        best = None
+        for i in range(3):
+            if f[i] == 0:
+                continue
+            score = (e[i] > 0, f[i] > 1, -f[i])
+            if best is None or score < best[0]:
+                best = (score, i)
+        return best[1]
+
+    def choose_bra(self, e):
+        best = None
+        for i in range(3):
+            if e[i] == 0:
+                continue
+            score = (e[i] > 1, -e[i])
+            if best is None or score < best[0]:
+                best = (score, i)
+        return best[1]
+
+    def need(self, e, f, m):
+        key = (e, f, m)
+        if key in self.nodes:
+            return
+        if f == Z and e == Z:
+            self.nodes[key] = ("base", ())
+            return
+        if f == Z:
+            i = self.choose_bra(e)
+            e1 = sub(e, i)
+            deps = [("PA", i, (e1, Z, m)), ("WP", i, (e1, Z, m + 1))]
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                deps.append(("Z2", e1[i], (e2, Z, m), (e2, Z, m + 1)))
+            self.nodes[key] = ("bra", (i, deps))
+        else:
+            i = self.choose_ket(e, f)
+            f1 = sub(f, i)
+            deps = [("QC", i, (e, f1, m)), ("WQ", i, (e, f1, m + 1))]
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                deps.append(("N2", f1[i], (e, f2, m), (e, f2, m + 1)))
+            if e[i] > 0:
+                deps.append(("ZN", e[i], (sub(e, i), f1, m + 1)))
+            self.nodes[key] = ("ket", (i, deps))
+        for d in self.nodes[key][1][1] if self.nodes[key][0] != "base" else ():
+            for x in d[2:]:
+                self.need(*x)
+
+    def order(self):
+        return sorted(self.nodes, key=lambda k: (am(k[0]) + am(k[1]), am(k[1]), -k[2], k))
+
+    @staticmethod
+    def var(k):
+        e, f, m = k
+        return "v%s_%s_%d" % (tag(e), tag(f), m)
+
+    def emit(self, acc_index):
+        lines = []
+        flops = 0
+        X = "xyz"
+        for k in self.order():
+            kind, info = self.nodes[k]
+            v = self.var(k)
+            if kind == "base":
+                lines.append("const R %s = F[%d];" % (v, k[2]))
+                continue
+            i, deps = info
+            terms = []
+            if kind == "bra":
+                terms.append("p.PA[%d] * %s" % (i, self.var(deps[0][2])))
+                terms.append("p.WP[%d] * %s" % (i, self.var(deps[1][2])))
+                flops += 3
+                if len(deps) > 2:
+                    n, a, b = deps[2][1], deps[2][2], deps[2][3]
+                    terms.append("R(%d) * p.oo2z * (%s - p.rz * %s)" % (n, self.var(a), self.var(b)))
+                    flops += 4 if n > 1 else 3
+            else:
+                terms.append("p.QC[%d] * %s" % (i, self.var(deps[0][2])))
+                terms.append("p.WQ[%d] * %s" % (i, self.var(deps[1][2])))
+                flops += 3
+                for d in deps[2:]:
+                    if d[0] == "N2":
+                        terms.append("R(%d) * p.oo2n * (%s - p.rn * %s)" % (d[1], self.var(d[2]), self.var(d[3])))
+                        flops += 4 if d[1] > 1 else 3
+                    else:
+                        terms.append("R(%d) * p.oo2zn * %s" % (d[1], self.var(d[2])))
+                        flops += 3 if d[1] > 1 else 2
+            lines.append("const R %s = %s;" % (v, " + ".join(terms)))
+        for (e, f) in self.targets:
+            lines.append("acc[%d] += %s;" % (acc_index[(e, f)], self.var((e, f, 0))))
+            flops += 1
+        return lines, flops
+
+
+class Hrr:
+    def __init__(self, La, Lb, Lc, Ld):
+        self.La, self.Lb, self.Lc, self.Ld = La, Lb, Lc, Ld
+        self.ket = {}    # (e, c, d) -> expr
+        self.bra = {}    # (a, b, c, d) -> expr
+        self.order_k = []
+        self.order_b = []
+
+    def need_ket(self, e, c, d):
+        k = (e, c, d)
+        if k in self.ket:
+            return
+        if d == Z:
+            self.ket[k] = None
+            return
+        i = next(j for j in range(3) if d[j] > 0)
+        self.need_ket(e, add(c, i), sub(d, i))
+        self.need_ket(e, c, sub(d, i))
+        self.ket[k] = i
+        self.order_k.append(k)
+
+    def need_bra(self, a, b, c, d):
+        k = (a, b, c, d)
+        if k in self.bra:
+            return
+        if b == Z:
+            self.need_ket(a, c, d)
+            self.bra[k] = None
+            return
+        i = next(j for j in range(3) if b[j] > 0)
+        self.need_bra(add(a, i), sub(b, i), c, d)
+        self.need_bra(a, sub(b, i), c, d)
+        self.bra[k] = i
+        self.order_b.append(k)
+
+    def emit(self, acc_index):
+        outs = []
+        for a in comps(self.La):
+            for b in comps(self.Lb):
+                for c in comps(self.Lc):
+                    for d in comps(self.Ld):
+                        self.need_bra(a, b, c, d)
+                        outs.append((a, b, c, d))
+        lines = []
+        flops = 0
+
+        def kv(e, c, d):
+            if d == Z:
+                return "acc[%d]" % acc_index[(e, c)]
+            return "k%s_%s_%s" % (tag(e), tag(c), tag(d))
+
+        def bv(a, b, c, d):
+            if b == Z:
+                return kv(a, c, d)
+            return "b%s_%s_%s_%s" % (tag(a), tag(b), tag(c), tag(d))
+
+        for (e, c, d) in self.order_k:
+            i = self.ket[(e, c, d)]
+            lines.append("const R %s = %s + CD[%d] * %s;" % (kv(e, c, d), kv(e, add(c, i), sub(d, i)), i,
+                                                            kv(e, c, sub(d, i))))
+            flops += 2
+        for (a, b, c, d) in self.order_b:
+            i = self.bra[(a, b, c, d)]
+            lines.append("const R %s = %s + AB[%d] * %s;" % (bv(a, b, c, d), bv(add(a, i), sub(b, i), c, d), i,
+                                                            bv(a, sub(b, i), c, d)))
+            flops += 2
+        for n, (a, b, c, d) in enumerate(outs):
+            lines.append("I[%d] = %s;" % (n, bv(a, b, c, d)))
+        return lines, flops
+
+
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+
+
+def quartet_classes():
+    out = []
+    for kb, (La, Lb) in enumerate(PAIR_CLASSES):
+        for kk, (Lc, Ld) in enumerate(PAIR_CLASSES):
+            if kk <= kb:
+                out.append((La, Lb, Lc, Ld))
+    return out
+
+
+def boys_setup_flop(L):
+    """Boys function (fit polynomial, exp, sqrt, recursion) plus the per
+    primitive-quartet setup (1/(z+n), PQ, T, prefactor, PA/WP/QC/WQ):
+    a fixed convention, 58 + 4 L."""
+    return 58 + 4 * L
+
+
+def ncart(L):
+    return (L + 1) * (L + 2) // 2
+
+
+def gen_class(La, Lb, Lc, Ld):
+    name = "C%d%d%d%d" % (La, Lb, Lc, Ld)
+    acc_index = {}
+    for E in range(La, La + Lb + 1):
+        for e in comps(E):
+            for F in range(Lc, Lc + Ld + 1):
+                for f in comps(F):
+                    acc_index[(e, f)] = len(acc_index)
+    vrr = Vrr(La, Lb, Lc, Ld)
+    vlines, vflops = vrr.emit(acc_index)
+    hrr = Hrr(La, Lb, Lc, Ld)
+    hlines, hflops = hrr.emit(acc_index)
+    nacc = len(acc_index)
+    ni = ncart(La) * ncart(Lb) * ncart(Lc) * ncart(Ld)
+    L = La + Lb + Lc + Ld
+    src = []
+    src.append("// (%s%s|%s%s): %d accumulators, %d integrals, VRR %d FLOP/prim, HRR %d FLOP"
+               % ("spd"[La], "spd"[Lb], "spd"[Lc], "spd"[Ld], nacc, ni, vflops, hflops))
+    src.append("struct %s {" % name)
+    src.append("  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L))
+    src.append("  static constexpr int NACC = %d, NI = %d;" % (nacc, ni))
+    src.append("  static constexpr int VRR_FLOP = %d, HRR_FLOP = %d;" % (vflops, hflops))
+    src.append("  template <typename R>")
+    src.append("  __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
+    src += ["    " + l for l in vlines]
+    src.append("  }")
+    src.append("  template <typename R>")
+    src.append("  __device__ static __forceinline__ void hrr(const R (&acc)[NACC], const R (&AB)[3], const R (&CD)[3], R (&I)[NI]) {")
+    src += ["    " + l for l in hlines]
+    src.append("  }")
+    src.append("};")
+    return name, "\n".join(src) + "\n", dict(La=La, Lb=Lb, Lc=Lc, Ld=Ld, nacc=nacc, ni=ni,
+                                             vrr_flop=vflops, hrr_flop=hflops, nodes=len(vrr.nodes))
+
+
+def launcher_source(name, diag):
+    out = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.",
+           '#include "%s.h"' % name, "", "namespace qf {", "",
+           "void launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
+           " hipStream_t s)" % name, "{",
+           "    if (nt <= 0) return;",
+           "    if (fp32)",
+           "        hipLaunchKernelGGL((fock_kernel<%s, float>), dim3(nt), dim3(WAVE), 0, s, t, b, m);" % name,
+           "    else",
+           "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(nt), dim3(WAVE), 0, s, t, b, m);" % name,
+           "}", ""]
+    if diag:
+        out += ["void launch_schwarz_%s(PairBlock blk, double* out, hipStream_t s)" % name, "{",
+                "    if (blk.n <= 0) return;",
+                "    hipLaunchKernelGGL((schwarz_kernel<%s>), dim3((blk.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s,"
+                " blk, out);" % name,
+                "}", ""]
+    out += ["}  // namespace qf", ""]
+    return "\n".join(out)
+
+
+def _write(path, text):
+    """Write only when the content changes, so incremental builds stay incremental."""
+    if os.path.exists(path):
+        with open(path) as fh:
+            if fh.read() == text:
+                return
+    with open(path, "w") as fh:
+        fh.write(text)
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    table = {}
+    names = []
+    for (La, Lb, Lc, Ld) in quartet_classes():
+        name, src, info = gen_class(La, Lb, Lc, Ld)
+        hdr = ("// Generated by quick_amd/codegen/gen_eri.py - do not edit.\n"
+               "#pragma once\n#include \"../qf_device.h\"\nnamespace qf {\n" + src + "}  // namespace qf\n")
+        _write(os.path.join(OUT, name + ".h"), hdr)
+        diag = (La, Lb) == (Lc, Ld)
+        _write(os.path.join(OUT, "eri_" + name + ".hip"), launcher_source(name, diag))
+        names.append((name, La, Lb, Lc, Ld, diag))
+        info["boys_setup_flop"] = boys_setup_flop(La + Lb + Lc + Ld)
+        info["prim_flop"] = info["vrr_flop"] + info["boys_setup_flop"]
+        info["quartet_flop"] = info["hrr_flop"] + 12 * info["ni"]
+        table[name] = info
+    # launch table
+    pc = {pcl: k for k, pcl in enumerate(PAIR_CLASSES)}
+    lines = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.", "#pragma once",
+             "#include <hip/hip_runtime.h>", '#include "../qf_types.h"', "", "namespace qf {", "",
+             "using FockLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);",
+             "using SchwarzLaunch = void (*)(PairBlock, double*, hipStream_t);", ""]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        lines.append("void launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);"
+                     % name)
+        if diag:
+            lines.append("void launch_schwarz_%s(PairBlock, double*, hipStream_t);" % name)
+    # FLOP model (FMA = 2): per surviving primitive quartet the VRR plus
+    # BOYS_SETUP_FLOP(L) for the Boys function and the quartet setup, per
+    # surviving contracted quartet the HRR plus 12 per Cartesian integral
+    # (six Fock updates).
+    lines += ["", "// Algorithmic FLOPs of nq quartets with np primitive quartets in class (cb|ck).",
+              "inline double class_flops(int cb, int ck, unsigned long long nq, unsigned long long np)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        info = table[name]
+        L = La + Lb + Lc + Ld
+        lines.append("    if (cb == %d && ck == %d) return double(np) * %d.0 + double(nq) * %d.0;"
+                     % (pc[(La, Lb)], pc[(Lc, Ld)], info["vrr_flop"] + boys_setup_flop(L),
+                        info["hrr_flop"] + 12 * info["ni"]))
+    lines += ["    return 0.0;", "}", ""]
+    lines += ["", "// indexed [bra pair class][ket pair class], pair class = La*(La+1)/2 + Lb",
+              "inline FockLaunch fock_launcher(int cb, int ck)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        lines.append("    if (cb == %d && ck == %d) return launch_fock_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
+    lines += ["    return nullptr;", "}", "", "inline SchwarzLaunch schwarz_launcher(int c)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        if diag:
+            lines.append("    if (c == %d) return launch_schwarz_%s;" % (pc[(La, Lb)], name))
+    lines += ["    return nullptr;", "}", "", "}  // namespace qf", ""]
+    _write(os.path.join(OUT, "launchers.h"), "\n".join(lines))
+    _write(os.path.join(OUT, "flop_table.json"), json.dumps(table, indent=1))
+    for k, v in table.items():
+        print(k, v)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

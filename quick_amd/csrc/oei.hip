@@ -1,0 +1,205 @@
+// One-electron matrices S, T, V for the SCF harness around the Fock build
+// (QUICK: quick_overlap_module.f90:172-363, quick_oei_module.f90:69-257,
+// attrashell.fh).  One thread per basis-function pair, McMurchie-Davidson
+// Hermite expansion; the nuclear-attraction Boys function is QUICK's FmT
+// called with Maxm = Qfinal(I) + Qfinal(J) as attrashell.fh:46-48,89.
+// Runs once per geometry, so it favours simplicity over speed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/quickfock.h"
+#include "qf_device.h"
+
+namespace {
+
+struct OeiFn {
+    int shell, L, lx, ly, lz, pstart, nprim, lmaxs;
+    double cons;
+    double X, Y, Z;
+};
+
+// Hermite coefficients E^{i,j}_t (i <= 2, j <= 4, t <= 6) in one direction,
+// without the exp(-mu X_AB^2) factor.
+__device__ void hermite_table(int la, int lb, double XPA, double XPB, double oo2p, double (&E)[3][5][7])
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 5; ++j)
+            for (int t = 0; t < 7; ++t) E[i][j][t] = 0.0;
+    E[0][0][0] = 1.0;
+    for (int i = 0; i <= la; ++i)
+        for (int j = 0; j <= lb; ++j) {
+            if (i == 0 && j == 0) continue;
+            for (int t = 0; t <= i + j; ++t) {
+                double v;
+                if (i > 0) {
+                    v = XPA * E[i - 1][j][t] + (t + 1 <= 6 ? (t + 1) * E[i - 1][j][t + 1] : 0.0);
+                    if (t > 0) v += oo2p * E[i - 1][j][t - 1];
+                } else {
+                    v = XPB * E[i][j - 1][t] + (t + 1 <= 6 ? (t + 1) * E[i][j - 1][t + 1] : 0.0);
+                    if (t > 0) v += oo2p * E[i][j - 1][t - 1];
+                }
+                E[i][j][t] = v;
+            }
+        }
+}
+
+__device__ void boys_rt(double X, int maxm, int L, double (&F)[5])
+{
+    // same algorithm as qf::boys_quick, with a run-time order (L <= 4)
+    double G[5];
+    switch (L) {
+        case 0: { double H[1]; qf::boys_quick<0>(X, maxm, H); G[0] = H[0]; break; }
+        case 1: { double H[2]; qf::boys_quick<1>(X, maxm, H); for (int m = 0; m < 2; ++m) G[m] = H[m]; break; }
+        case 2: { double H[3]; qf::boys_quick<2>(X, maxm, H); for (int m = 0; m < 3; ++m) G[m] = H[m]; break; }
+        case 3: { double H[4]; qf::boys_quick<3>(X, maxm, H); for (int m = 0; m < 4; ++m) G[m] = H[m]; break; }
+        default: { double H[5]; qf::boys_quick<4>(X, maxm, H); for (int m = 0; m < 5; ++m) G[m] = H[m]; break; }
+    }
+    for (int m = 0; m < 5; ++m) F[m] = m <= L ? G[m] : 0.0;
+}
+
+__global__ void k_oei(const OeiFn* __restrict__ fn, int n, const double* __restrict__ exps,
+                      const double* __restrict__ coef, int natom, const double* __restrict__ xyz,
+                      const double* __restrict__ charge, double* __restrict__ S, double* __restrict__ T,
+                      double* __restrict__ V)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int mu = idx / n, nu = idx % n;
+    const OeiFn a = fn[mu], b = fn[nu];
+    const int la[3] = {a.lx, a.ly, a.lz}, lb[3] = {b.lx, b.ly, b.lz};
+    const double A[3] = {a.X, a.Y, a.Z}, B[3] = {b.X, b.Y, b.Z};
+    const int L = a.L + b.L;
+    const int maxm = a.lmaxs + b.lmaxs;
+    double AB2 = 0.0;
+    for (int d = 0; d < 3; ++d) AB2 += (A[d] - B[d]) * (A[d] - B[d]);
+    double s_sum = 0.0, t_sum = 0.0, v_sum = 0.0;
+    for (int i = 0; i < a.nprim; ++i)
+        for (int j = 0; j < b.nprim; ++j) {
+            const double al = exps[a.pstart + i], be = exps[b.pstart + j];
+            const double ca = coef[(a.pstart + i) * 4 + a.L] * a.cons;
+            const double cb = coef[(b.pstart + j) * 4 + b.L] * b.cons;
+            const double p = al + be, oo2p = 0.5 / p;
+            const double Kab = exp(-al * be / p * AB2);
+            double P[3], E[3][3][5][7];
+            for (int d = 0; d < 3; ++d) {
+                P[d] = (al * A[d] + be * B[d]) / p;
+                hermite_table(la[d], lb[d] + 2, P[d] - A[d], P[d] - B[d], oo2p, E[d]);
+            }
+            const double pre = pow(3.14159265358979323846 / p, 1.5) * Kab;
+            double s1[3], t1[3];
+            for (int d = 0; d < 3; ++d) {
+                const int ai = la[d], bj = lb[d];
+                s1[d] = E[d][ai][bj][0];
+                double tk = -2.0 * be * be * E[d][ai][bj + 2][0] + be * (2 * bj + 1) * E[d][ai][bj][0];
+                if (bj >= 2) tk -= 0.5 * bj * (bj - 1) * E[d][ai][bj - 2][0];
+                t1[d] = tk;
+            }
+            s_sum += ca * cb * pre * s1[0] * s1[1] * s1[2];
+            t_sum += ca * cb * pre * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
+            // nuclear attraction: -Z 2pi/p K_AB sum_tuv E^x_t E^y_u E^z_v R_tuv(P - C)
+            for (int c = 0; c < natom; ++c) {
+                const double PC[3] = {P[0] - xyz[3 * c], P[1] - xyz[3 * c + 1], P[2] - xyz[3 * c + 2]};
+                const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                double F[5];
+                boys_rt(Tb, maxm, L, F);
+                // R^N_tuv level by level, N = L..0
+                double R[5][5][5], Rn[5][5][5];
+                for (int N = L; N >= 0; --N) {
+                    double f = F[N];
+                    for (int k = 0; k < N; ++k) f *= -2.0 * p;
+                    for (int t = 0; t <= L - N; ++t)
+                        for (int u = 0; u <= L - N - t; ++u)
+                            for (int v = 0; v <= L - N - t - u; ++v) {
+                                double r;
+                                if (t + u + v == 0) {
+                                    r = f;
+                                } else if (t > 0) {
+                                    r = PC[0] * R[t - 1][u][v] + (t > 1 ? (t - 1) * R[t - 2][u][v] : 0.0);
+                                } else if (u > 0) {
+                                    r = PC[1] * R[t][u - 1][v] + (u > 1 ? (u - 1) * R[t][u - 2][v] : 0.0);
+                                } else {
+                                    r = PC[2] * R[t][u][v - 1] + (v > 1 ? (v - 1) * R[t][u][v - 2] : 0.0);
+                                }
+                                Rn[t][u][v] = r;
+                            }
+                    for (int t = 0; t <= L - N; ++t)
+                        for (int u = 0; u <= L - N - t; ++u)
+                            for (int v = 0; v <= L - N - t - u; ++v) R[t][u][v] = Rn[t][u][v];
+                }
+                double sum = 0.0;
+                for (int t = 0; t <= la[0] + lb[0]; ++t)
+                    for (int u = 0; u <= la[1] + lb[1]; ++u)
+                        for (int v = 0; v <= la[2] + lb[2]; ++v)
+                            sum += E[0][la[0]][lb[0]][t] * E[1][la[1]][lb[1]][u] * E[2][la[2]][lb[2]][v] * R[t][u][v];
+                v_sum -= charge[c] * ca * cb * 2.0 * 3.14159265358979323846 / p * Kab * sum;
+            }
+        }
+    S[idx] = s_sum;
+    T[idx] = t_sum;
+    V[idx] = v_sum;
+}
+
+}  // namespace
+
+// The engine's basis is reachable through a small accessor declared in engine.hip.
+extern "C" int qf_internal_basis(qf_engine* e, int* natom, const double** xyz, int* nshell, const int** s_atom,
+                                 const int** s_lmin, const int** s_lmax, const int** s_nprim, const int** s_pstart,
+                                 const int** s_fstart, int* nprim, const double** exps, const double** coef,
+                                 int* nbasis);
+
+extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, double* T, double* V)
+{
+    int natom, nshell, nprim, nbasis;
+    const double *xyz, *exps, *coef;
+    const int *s_atom, *s_lmin, *s_lmax, *s_nprim, *s_pstart, *s_fstart;
+    int rc = qf_internal_basis(e, &natom, &xyz, &nshell, &s_atom, &s_lmin, &s_lmax, &s_nprim, &s_pstart, &s_fstart,
+                               &nprim, &exps, &coef, &nbasis);
+    if (rc) return rc;
+    static const int comp[3][6][3] = {{{0, 0, 0}},
+                                      {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+                                      {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}}};
+    const double s3 = std::sqrt(3.0);
+    const double cons[3][6] = {{1}, {1, 1, 1}, {1, s3, 1, s3, s3, 1}};
+    std::vector<OeiFn> fn(nbasis);
+    for (int s = 0; s < nshell; ++s) {
+        int f = s_fstart[s];
+        for (int L = s_lmin[s]; L <= s_lmax[s]; ++L)
+            for (int k = 0; k < (L + 1) * (L + 2) / 2; ++k, ++f) {
+                const double* X = xyz + 3 * s_atom[s];
+                fn[f] = OeiFn{s, L, comp[L][k][0], comp[L][k][1], comp[L][k][2], s_pstart[s], s_nprim[s], s_lmax[s],
+                              cons[L][k], X[0], X[1], X[2]};
+            }
+    }
+    const size_t n2 = (size_t)nbasis * nbasis;
+    OeiFn* dfn;
+    double *dex, *dco, *dxyz, *dch, *dS, *dT, *dV;
+    if (hipMalloc(&dfn, sizeof(OeiFn) * nbasis) != hipSuccess) return QF_ERR_HIP;
+    (void)hipMalloc(&dex, sizeof(double) * nprim);
+    (void)hipMalloc(&dco, sizeof(double) * 4 * nprim);
+    (void)hipMalloc(&dxyz, sizeof(double) * 3 * natom);
+    (void)hipMalloc(&dch, sizeof(double) * natom);
+    (void)hipMalloc(&dS, sizeof(double) * n2 * 3);
+    dT = dS + n2;
+    dV = dT + n2;
+    (void)hipMemcpy(dfn, fn.data(), sizeof(OeiFn) * nbasis, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dex, exps, sizeof(double) * nprim, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dco, coef, sizeof(double) * 4 * nprim, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dxyz, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dch, charges, sizeof(double) * natom, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_oei, dim3((n2 + 127) / 128), dim3(128), 0, 0, dfn, nbasis, dex, dco, natom, dxyz, dch, dS,
+                       dT, dV);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(S, dS, sizeof(double) * n2, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(T, dT, sizeof(double) * n2, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(V, dV, sizeof(double) * n2, hipMemcpyDeviceToHost);
+    (void)hipFree(dfn);
+    (void)hipFree(dex);
+    (void)hipFree(dco);
+    (void)hipFree(dxyz);
+    (void)hipFree(dch);
+    (void)hipFree(dS);
+    return err == hipSuccess ? QF_OK : QF_ERR_HIP;
+}

@@ -1,0 +1,339 @@
+// Device-side building blocks of the Fock build: QUICK-compatible Boys
+// function, per-primitive-quartet VRR parameters, and the two kernel
+// templates (Fock digestion and Schwarz diagonal) that every generated
+// (La Lb | Lc Ld) class instantiates.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qf_types.h"
+
+namespace qf {
+
+template <typename R>
+struct VrrPar {
+    R PA[3], WP[3], QC[3], WQ[3];
+    R oo2z, oo2n, oo2zn, rz, rn;   // 1/2z, 1/2n, 1/2(z+n), rho/z, rho/n
+};
+
+// ---------------------------------------------------------------------------
+// Boys function F_m(X), m = 0..L, evaluated exactly as QUICK's FmT
+// (src/subs/fmt.f90:13-85, src/gpu/gpu_fmt.h): the same piecewise fits for
+// F_0, the same branch limits, upward recursion above X = 0.1 (or 1e-4 when
+// maxm < 4), otherwise downward recursion from (1-X)/(2 maxm+1).  maxm is
+// QUICK's NABCD: the sum of the four *original* shells' highest L (an SP
+// shell counts 1 even for its s part), which selects the small-X branch.
+// The fit coefficients are QUICK's; parity with QUICK needs them.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1])
+{
+    const double XINV = 1.0 / X;
+    const double E = exp(-X);
+    const bool up = (X > 1.0e-1) || (X > 1.0e-4 && maxm < 4);
+    double WW1;
+    if (X > 5.0) {
+        const double s = sqrt(0.78539816339744830962 * XINV);
+        if (X > 33.0) {
+            WW1 = s;
+        } else if (X > 15.0) {
+            WW1 = ((1.9623264149430e-01 * XINV - 4.9695241464490e-01) * XINV - 6.0156581186481e-05) * E + s;
+        } else if (X > 10.0) {
+            WW1 = (((-1.8784686463512e-01 * XINV + 2.2991849164985e-01) * XINV - 4.9893752514047e-01) * XINV -
+                   2.1916512131607e-05) * E + s;
+        } else {
+            WW1 = ((((((4.6897511375022e-01 * XINV - 6.9955602298985e-01) * XINV + 5.3689283271887e-01) * XINV -
+                       3.2883030418398e-01) * XINV + 2.4645596956002e-01) * XINV - 4.9984072848436e-01) * XINV -
+                   3.1501078774085e-06) * E + s;
+        }
+    } else if (X > 1.0) {
+        double F1;
+        if (X > 3.0) {
+            const double Y = X - 4.0;
+            F1 = ((((((((((-2.62453564772299e-11 * Y + 3.24031041623823e-10) * Y - 3.614965656163e-09) * Y +
+                        3.760256799971e-08) * Y - 3.553558319675e-07) * Y + 3.022556449731e-06) * Y -
+                      2.290098979647e-05) * Y + 1.526537461148e-04) * Y - 8.81947375894379e-04) * Y +
+                   4.33207949514611e-03) * Y - 1.75257821619926e-02) * Y + 5.28406320615584e-02;
+        } else {
+            const double Y = X - 2.0;
+            F1 = ((((((((((-1.61702782425558e-10 * Y + 1.96215250865776e-09) * Y - 2.14234468198419e-08) * Y +
+                        2.17216556336318e-07) * Y - 1.98850171329371e-06) * Y + 1.62429321438911e-05) * Y -
+                      1.16740298039895e-04) * Y + 7.24888732052332e-04) * Y - 3.79490003707156e-03) * Y +
+                   1.61723488664661e-02) * Y - 5.29428148329736e-02) * Y + 1.15702180856167e-01;
+        }
+        WW1 = (X + X) * F1 + E;
+    } else if (up) {
+        const double F1 = ((((((((-8.36313918003957e-08 * X + 1.21222603512827e-06) * X - 1.15662609053481e-05) * X +
+                               9.25197374512647e-05) * X - 6.40994113129432e-04) * X + 3.78787044215009e-03) * X -
+                             1.85185172458485e-02) * X + 7.14285713298222e-02) * X - 1.99999999997023e-01) * X +
+                          3.33333333333318e-01;
+        WW1 = (X + X) * F1 + E;
+    } else {
+        WW1 = (1.0 - X) / (double)(2 * maxm + 1);
+    }
+    if (up) {
+        F[0] = WW1;
+        const double h = 0.5 * XINV;
+#pragma unroll
+        for (int m = 1; m <= L; ++m) F[m] = ((2 * m - 1) * F[m - 1] - E) * h;
+    } else {
+        const double twot = X * 2.0;
+        double f = WW1;
+        for (int m = maxm - 1; m >= L; --m) f = (twot * f + E) / (double)(m * 2 + 1);
+        F[L] = f;
+#pragma unroll
+        for (int m = L - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) / (double)(m * 2 + 1);
+    }
+}
+
+// cons factors of QUICK's Cartesian components (basis.f90:567-640)
+__device__ constexpr double kCons[3][6] = {
+    {1.0, 0, 0, 0, 0, 0},
+    {1.0, 1.0, 1.0, 0, 0, 0},
+    {1.0, 1.7320508075688772935, 1.0, 1.7320508075688772935, 1.7320508075688772935, 1.0}};
+
+constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// Contracted [e0|f0] for one (bra pair, ket pair): primitive loops with
+// QUICK's primitive screen dnmax*cutprim(ij)*cutprim(kl) > primLimit
+// (eri.fh:270,288-289).  Primitive pairs are sorted by cutprim descending,
+// so the first failing ket primitive ends the inner loop and a failing
+// first ket primitive ends the outer loop: the same set QUICK skips.
+// ---------------------------------------------------------------------------
+template <class C, typename R>
+__device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
+                                         int j, const PairInfo& ki, double dn, double primlim, bool screen,
+                                         int maxm, R (&acc)[C::NACC])
+{
+    const double* __restrict__ bp = bb.prim;
+    const double* __restrict__ kp = kb.prim;
+    const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
+    const double kcut0 = kp[(PF_CUT * kK) * kn + j];
+    for (int ip = 0; ip < bK; ++ip) {
+        const double zb = bp[(PF_ZETA * bK + ip) * bn + i];
+        const double ozb = bp[(PF_OOZ2 * bK + ip) * bn + i];
+        const double Pb[3] = {bp[(PF_PX * bK + ip) * bn + i], bp[(PF_PY * bK + ip) * bn + i],
+                              bp[(PF_PZ * bK + ip) * bn + i]};
+        const double Kb = bp[(PF_KC * bK + ip) * bn + i];
+        const double c1 = dn * bp[(PF_CUT * bK + ip) * bn + i];
+        if (screen && !(c1 * kcut0 > primlim)) break;
+        for (int jp = 0; jp < kK; ++jp) {
+            if (screen && !(c1 * kp[(PF_CUT * kK + jp) * kn + j] > primlim)) break;
+            const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
+            const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
+            const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
+                                  kp[(PF_PZ * kK + jp) * kn + j]};
+            const double Kk = kp[(PF_KC * kK + jp) * kn + j];
+            const double inv = 1.0 / (zb + zk);
+            const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
+            const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+            const double T = rpq * (zb * zk * inv);
+            double F[C::L + 1];
+            boys_quick<C::L>(T, maxm, F);
+            const double pref = Kb * Kk * sqrt(inv);
+            R Fr[C::L + 1];
+#pragma unroll
+            for (int m = 0; m <= C::L; ++m) Fr[m] = R(F[m] * pref);
+            VrrPar<R> p;
+            const double wp = zk * inv, wq = zb * inv;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                p.PA[d] = R(Pb[d] - bi.A[d]);
+                p.WP[d] = R(-wp * PQ[d]);
+                p.QC[d] = R(Qk[d] - ki.A[d]);
+                p.WQ[d] = R(wq * PQ[d]);
+            }
+            p.oo2z = R(ozb);
+            p.oo2n = R(ozk);
+            p.oo2zn = R(0.5 * inv);
+            p.rz = R(wp);
+            p.rn = R(wq);
+            C::template vrr<R>(Fr, p, acc);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fock kernel: one wave per task = one bra pair x up to 64 ket pairs of one
+// block.  Screening per lane as getCshellEri (eri.fh:173-189).  Digestion of
+// the canonical quartet with degeneracy weight s = f/4,
+// f = (a!=b ? 2:1)(c!=d ? 2:1)(ab!=cd ? 2:1):
+//   Gt_ab += s D_cd I, Gt_cd += s D_ab I,
+//   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc, Gt_bc -= .. D_ad, Gt_bd -= .. D_ac,
+// and G = Gt + Gt^T afterwards.  The bra block Gt_ab is summed over the wave
+// before its atomics; the other five go straight to FP64 atomics.
+// ---------------------------------------------------------------------------
+template <class C, typename R>
+__global__ __launch_bounds__(WAVE) void fock_kernel(const Task* __restrict__ tasks,
+                                                    const PairBlock* __restrict__ blocks,
+                                                    const MolDesc* __restrict__ mols)
+{
+    constexpr int na = ncart(C::La), nb = ncart(C::Lb), nc = ncart(C::Lc), nd = ncart(C::Ld);
+    const Task t = tasks[blockIdx.x];
+    const PairBlock bb = blocks[t.bra_block];
+    const PairBlock kb = blocks[t.ket_block];
+    const MolDesc md = mols[bb.mol];
+    const int lane = threadIdx.x;
+    const int i = t.bra_idx;
+    const PairInfo bi = bb.info[i];
+    bool active = lane < t.ket_count;
+    const int j = active ? t.ket_start + lane : t.ket_start;
+    const PairInfo ki = kb.info[j];
+
+    const int ns = md.nshell;
+    const double* __restrict__ cm = md.cutmat;
+    double dn = 4.0 * cm[bi.sa * ns + bi.sb];
+    dn = fmax(dn, 4.0 * cm[ki.sa * ns + ki.sb]);
+    dn = fmax(dn, cm[bi.sa * ns + ki.sb]);
+    dn = fmax(dn, cm[bi.sa * ns + ki.sa]);
+    dn = fmax(dn, cm[bi.sb * ns + ki.sa]);
+    dn = fmax(dn, cm[bi.sb * ns + ki.sb]);
+    const double ytest = bi.ycut * ki.ycut;
+    active = active && (ytest > md.cutoff) && (ytest * dn > md.cutoff);
+
+    R acc[C::NACC];
+#pragma unroll
+    for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
+    if (active) contract<C, R>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
+
+    R I[C::NI];
+    {
+        const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
+        const R CD[3] = {R(ki.AB[0]), R(ki.AB[1]), R(ki.AB[2])};
+        C::template hrr<R>(acc, AB, CD, I);
+    }
+
+    const int n = md.nbasis;
+    const double* __restrict__ P = md.P;
+    double* __restrict__ G = md.G;
+    const double s = active ? 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) *
+                                  (bi.id != ki.id ? 2.0 : 1.0)
+                            : 0.0;
+    const double sk = -0.25 * md.hyb * s;
+
+    R Dab[na * nb], Dcd[nc * nd], Dac[na * nc], Dad[na * nd], Dbc[nb * nc], Dbd[nb * nd];
+#pragma unroll
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int b = 0; b < nb; ++b) Dab[a * nb + b] = R(P[(bi.fa + a) * n + bi.fb + b]);
+#pragma unroll
+    for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = R(P[(ki.fa + c) * n + ki.fb + d]);
+#pragma unroll
+    for (int a = 0; a < na; ++a) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c) Dac[a * nc + c] = R(P[(bi.fa + a) * n + ki.fa + c]);
+#pragma unroll
+        for (int d = 0; d < nd; ++d) Dad[a * nd + d] = R(P[(bi.fa + a) * n + ki.fb + d]);
+    }
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c) Dbc[b * nc + c] = R(P[(bi.fb + b) * n + ki.fa + c]);
+#pragma unroll
+        for (int d = 0; d < nd; ++d) Dbd[b * nd + d] = R(P[(bi.fb + b) * n + ki.fb + d]);
+    }
+
+    R Jab[na * nb], Jcd[nc * nd], Kac[na * nc], Kad[na * nd], Kbc[nb * nc], Kbd[nb * nd];
+#pragma unroll
+    for (int k = 0; k < na * nb; ++k) Jab[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < nc * nd; ++k) Jcd[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < na * nc; ++k) Kac[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < na * nd; ++k) Kad[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < nb * nc; ++k) Kbc[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < nb * nd; ++k) Kbd[k] = R(0);
+
+#pragma unroll
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+                for (int d = 0; d < nd; ++d) {
+                    const R cons = R(kCons[C::La][a] * kCons[C::Lb][b] * kCons[C::Lc][c] * kCons[C::Ld][d]);
+                    const R v = I[((a * nb + b) * nc + c) * nd + d] * cons;
+                    Jab[a * nb + b] += Dcd[c * nd + d] * v;
+                    Jcd[c * nd + d] += Dab[a * nb + b] * v;
+                    Kac[a * nc + c] += Dbd[b * nd + d] * v;
+                    Kad[a * nd + d] += Dbc[b * nc + c] * v;
+                    Kbc[b * nc + c] += Dad[a * nd + d] * v;
+                    Kbd[b * nd + d] += Dac[a * nc + c] * v;
+                }
+
+    // bra block: wave-reduced, one atomic per element
+#pragma unroll
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+            const double v = wave_sum(s * double(Jab[a * nb + b]));
+            if (lane == 0 && v != 0.0) atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+        }
+    if (!active) return;
+#pragma unroll
+    for (int c = 0; c < nc; ++c)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], s * double(Jcd[c * nd + d]));
+#pragma unroll
+    for (int a = 0; a < na; ++a) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fa + a) * n + ki.fa + c], sk * double(Kac[a * nc + c]));
+#pragma unroll
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fa + a) * n + ki.fb + d], sk * double(Kad[a * nd + d]));
+    }
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fb + b) * n + ki.fa + c], sk * double(Kbc[b * nc + c]));
+#pragma unroll
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fb + b) * n + ki.fb + d], sk * double(Kbd[b * nd + d]));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz kernel for a diagonal class (La Lb | La Lb): one lane per pair,
+// out[j] = max over components |(ab|ab)| with no screening
+// (shellcutoff / classcutoff, quick_cutoff_module.f90:91-424).  Used with
+// contracted pairs (Ycutoff) and with single-primitive pairs (cutprim).
+// ---------------------------------------------------------------------------
+template <class C>
+__global__ __launch_bounds__(WAVE) void schwarz_kernel(PairBlock blk, double* __restrict__ out)
+{
+    constexpr int na = ncart(C::La), nb = ncart(C::Lb);
+    const int j = blockIdx.x * WAVE + threadIdx.x;
+    if (j >= blk.n) return;
+    const PairInfo pi = blk.info[j];
+    double acc[C::NACC];
+#pragma unroll
+    for (int k = 0; k < C::NACC; ++k) acc[k] = 0.0;
+    contract<C, double>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc);
+    double I[C::NI];
+    const double AB[3] = {pi.AB[0], pi.AB[1], pi.AB[2]};
+    C::template hrr<double>(acc, AB, AB, I);
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+            const double cons = kCons[C::La][a] * kCons[C::Lb][b];
+            m = fmax(m, fabs(I[((a * nb + b) * na + a) * nb + b] * cons * cons));
+        }
+    out[j] = m;
+}
+
+}  // namespace qf

@@ -1,0 +1,106 @@
+"""Restricted Hartree-Fock around the GPU Fock build.
+
+Restates the control flow of QUICK's ``electdiis`` (quick_scf_module.f90:
+344-760) and ``scf_operator`` (quick_scf_operator_module.f90:26-250):
+
+* F = H + G(P); after ``ncyc`` full builds optionally incremental,
+  F = F_prev + G(P - P_prev) (``deltaO``, quick_scf_operator_module.f90:69-79),
+* E = 1/2 sum P (H + F) + E_nuc (eri.fh:1064-1083),
+* Pulay DIIS on the orthogonalised commutator X^T (F P S - S P F) X with up to
+  ``ndiis`` vectors, falling back to the plain Fock matrix when the DIIS
+  system is singular (quick_scf_module.f90:405-602),
+* X = S^-1/2, P = 2 C_occ C_occ^T.
+
+The Fock build runs on the GPU through :class:`quick_amd.engine.FockEngine`;
+diagonalisation and DIIS (O(n^3), FP64) use LAPACK on the host.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+
+def _orth(S):
+    s, U = np.linalg.eigh(S)
+    return (U / np.sqrt(s)) @ U.T
+
+
+class RHFResult(dict):
+    __getattr__ = dict.__getitem__
+
+
+def rhf(engine, nelec=None, maxiter=100, etol=1e-8, dtol=1e-6, ndiis=10, incremental=False, ncyc=3,
+        guess=None, one_electron=None, verbose=False):
+    mol = engine.basis.mol
+    if nelec is None:
+        nelec = mol.nelec()
+    if nelec % 2:
+        raise ValueError("RHF needs an even number of electrons")
+    nocc = nelec // 2
+    t0 = time.perf_counter()
+    S, T, V = one_electron if one_electron is not None else engine.one_electron()
+    H = T + V
+    X = _orth(S)
+    enuc = mol.nuclear_repulsion()
+
+    def density(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        return 2.0 * C[:, :nocc] @ C[:, :nocc].T, e
+
+    P = guess if guess is not None else density(H)[0]
+    fs, es, hist, ftimes = [], [], [], []
+    E_old = 0.0
+    F = None
+    P_prev = None
+    for it in range(maxiter):
+        tf = time.perf_counter()
+        if incremental and it >= ncyc and F is not None:
+            F = F + engine.fock2e(P - P_prev)       # F_prev already holds H + G(P_prev)
+        else:
+            F = H + engine.fock2e(P)
+        ftimes.append(time.perf_counter() - tf)
+        E = 0.5 * float(np.sum(P * (H + F))) + enuc
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        fs.append(F)
+        es.append(err)
+        fs, es = fs[-ndiis:], es[-ndiis:]
+        k = len(fs)
+        B = -np.ones((k + 1, k + 1))
+        B[k, k] = 0.0
+        for i in range(k):
+            for j in range(k):
+                B[i, j] = float(np.sum(es[i] * es[j]))
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:k]
+            Fd = sum(ci * fi for ci, fi in zip(c, fs))
+        except np.linalg.LinAlgError:
+            Fd = F
+        Pn, eps = density(Fd)
+        dP = float(np.sqrt(np.mean((Pn - P) ** 2)))
+        hist.append(E)
+        if verbose:
+            print(f"{it:3d} {E:.12f} {E - E_old:+.3e} rmsdP {dP:.3e} fock {ftimes[-1] * 1e3:.1f} ms", flush=True)
+        converged = abs(E - E_old) < etol and dP < dtol
+        E_old = E
+        P_prev = P
+        P = Pn
+        if converged:
+            return RHFResult(energy=E, density=P_prev, iterations=it + 1, history=hist, fock_times=ftimes,
+                             wall=time.perf_counter() - t0, converged=True, e_nuc=enuc)
+    return RHFResult(energy=E, density=P_prev, iterations=maxiter, history=hist, fock_times=ftimes,
+                     wall=time.perf_counter() - t0, converged=False, e_nuc=enuc)
+
+
+def seeded_density(S, nocc, seed=20261202):
+    """P = 2 C C^T with C (n x nocc) ~ N(0,1) from default_rng(seed), then
+    S-orthonormalised, C <- C (C^T S C)^(-1/2): rank nocc, tr(PS) = 2 nocc."""
+    n = S.shape[0]
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, nocc))
+    w, U = np.linalg.eigh(C.T @ S @ C)
+    C = C @ (U @ np.diag(w ** -0.5) @ U.T)
+    return 2.0 * C @ C.T

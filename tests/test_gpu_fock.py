@@ -1,0 +1,202 @@
+"""Parity of the HIP Fock build (through the C ABI) with the CPU oracle.
+
+Bar: FP64 Fock matrix max |dF| <= 1e-10 Eh against QUICK's CPU get2e
+restated (same geometry, basis, density, cutoffs); SCF energy <= 1e-8 Eh.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import announce, molecule, oracle_for, product_basis, seeded_density_for
+
+pytestmark = pytest.mark.gpu
+
+FOCK_TOL = 1e-10
+
+
+def _engine(bs, **kw):
+    from quick_amd.engine import FockEngine
+    return FockEngine(bs, **kw)
+
+
+CASES = [
+    ("water", "6-31G*", 1e-12),
+    ("water", "cc-pVDZ", 1e-12),
+    ("water", "STO-3G", 1e-12),
+    ("water3", "6-31G*", 1e-12),
+    ("alkane2", "6-31G**", 1e-10),
+    ("gly1", "6-31G*", 1e-11),
+    ("LiF", "3-21G", 1e-12),
+    ("NH4+", "def2-SVP", 1e-12),
+]
+
+
+@pytest.mark.parametrize("name,basis,cutoff", CASES)
+def test_schwarz_bounds_match_oracle(name, basis, cutoff):
+    announce(f"schwarz {name} {basis}")
+    mol = molecule(name)
+    orc = oracle_for(mol, basis)
+    eng = _engine(product_basis(mol, basis), cutoff=cutoff)
+    y_gpu, c_gpu = eng.schwarz()
+    y_ref, c_ref = orc.schwarz()
+    assert np.allclose(y_gpu, y_ref, rtol=1e-11, atol=1e-14)
+    mask = c_ref > 0          # QUICK fills cutprim for shell order II <= JJ only
+    assert np.allclose(c_gpu[mask], c_ref[mask], rtol=1e-11, atol=1e-14)
+    assert np.allclose(c_gpu, c_gpu.T)
+
+
+@pytest.mark.parametrize("name,basis,cutoff", CASES)
+def test_fock_matches_oracle(name, basis, cutoff):
+    announce(f"fock {name} {basis} cutoff={cutoff}")
+    mol = molecule(name)
+    orc = oracle_for(mol, basis)
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, nq = orc.fock2e(P, cutoff=cutoff, threads=4)
+    eng = _engine(product_basis(mol, basis), cutoff=cutoff)
+    G = eng.fock2e(P)
+    err = float(np.max(np.abs(G - G_ref)))
+    print(f"  max|dF| = {err:.3e}  (quartets {nq})", flush=True)
+    assert err <= FOCK_TOL
+    assert np.max(np.abs(G - G.T)) <= 1e-14 * max(1.0, np.max(np.abs(G)))
+
+
+def test_config1_water_fock_with_quick_schwarz_arrays():
+    """Drop-in path: Ycutoff/cutprim as QUICK computes them (one triangle of
+    cutprim filled) handed to the engine instead of the GPU's own."""
+    announce("config1 water 6-31G* with oracle Schwarz arrays")
+    mol = molecule("water")
+    orc = oracle_for(mol, "6-31G*")
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, _ = orc.fock2e(P, cutoff=1e-12)
+    eng = _engine(product_basis(mol, "6-31G*"), cutoff=1e-12, schwarz=orc.schwarz())
+    G = eng.fock2e(P)
+    assert np.max(np.abs(G - G_ref)) <= FOCK_TOL
+
+
+def test_one_electron_matrices_match_oracle():
+    announce("one-electron S T V water 6-31G* and def2-SVP NH4+")
+    for name, basis in [("water", "6-31G*"), ("NH4+", "def2-SVP"), ("gly1", "6-31G**")]:
+        mol = molecule(name)
+        orc = oracle_for(mol, basis)
+        eng = _engine(product_basis(mol, basis))
+        for a, b in zip(eng.one_electron(), orc.one_electron()):
+            assert np.max(np.abs(a - b)) < 1e-11
+
+
+def test_fock_is_linear_in_density_and_symmetric():
+    """Size-independent properties on the 16-water cluster: G(P1) + G(P2-P1)
+    = G(P2) (the incremental build QUICK uses after ncyc) and symmetry.
+    Screening uses the density, so compare with screening off."""
+    announce("linearity water16 6-31G*")
+    mol = molecule("water16")
+    eng = _engine(product_basis(mol, "6-31G*"), cutoff=1e-30, prim_limit=1e-31)
+    S, _, _ = eng.one_electron()
+    from quick_amd.scf import seeded_density
+    P1 = seeded_density(S, mol.nelec() // 2, seed=1)
+    P2 = seeded_density(S, mol.nelec() // 2, seed=2)
+    G1, G2, G12 = eng.fock2e(P1), eng.fock2e(P2), eng.fock2e(P2 - P1)
+    scale = np.max(np.abs(G2))
+    assert np.max(np.abs(G1 + G12 - G2)) < 1e-12 * scale
+    assert np.max(np.abs(G2 - G2.T)) < 1e-14 * scale
+
+
+def test_scf_energy_matches_oracle_config1():
+    """Converged RHF energy, GPU Fock builds vs oracle Fock builds, same
+    cutoff; and QUICK's saved value for its own H2O/6-31G* test geometry."""
+    announce("scf water 6-31G*")
+    from oracle import oracle as O
+    from quick_amd import basis as B
+    from quick_amd.scf import rhf
+    xyz = np.array([[-0.33840, 0.00380, 0.23923], [-0.33510, -0.00190, -0.83277], [0.67350, -0.00190, 0.59353]])
+    mol = B.Molecule.from_angstrom(["O", "H", "H"], xyz)
+    orc = oracle_for(mol, "6-31G*")
+    E_ref, _, _ = O.rhf(orc, 10, cutoff=1e-10, prim_limit=1e-10, etol=1e-11, dtol=1e-9)
+    eng = _engine(product_basis(mol, "6-31G*"), cutoff=1e-10, prim_limit=1e-10)
+    res = rhf(eng, etol=1e-11, dtol=1e-9)
+    assert res.converged
+    print(f"  E_gpu {res.energy:.12f}  E_oracle {E_ref:.12f}", flush=True)
+    assert abs(res.energy - E_ref) <= 1e-8
+    assert abs(res.energy - (-75.983996360)) < 3e-8
+
+
+def test_mixed_precision_error_is_bounded():
+    announce("mixed precision water8 6-31G*")
+    mol = molecule("water8")
+    bs = product_basis(mol, "6-31G*")
+    e64 = _engine(bs, cutoff=1e-12)
+    S, _, _ = e64.one_electron()
+    from quick_amd.scf import seeded_density
+    P = seeded_density(S, mol.nelec() // 2)
+    G64 = e64.fock2e(P)
+    emx = _engine(bs, cutoff=1e-12, precision="mixed", split=1e-4)
+    Gmx = emx.fock2e(P)
+    st = emx.stats()
+    assert st["n_tasks_fp32"] > 0
+    dF = float(np.max(np.abs(Gmx - G64)))
+    dE = 0.5 * abs(float(np.sum(P * (Gmx - G64))))
+    print(f"  mixed: max|dF| {dF:.2e}  |dE2| {dE:.2e} Eh", flush=True)
+    assert dF < 1e-6
+    assert dE < 1e-6
+
+
+def test_quick_fortran_abi_drop_in():
+    """Call the gpu_*_ entry points the way QUICK's Fortran does: 1-based
+    shell arrays, gccoeff/gcexpo(MAXPRIM=14, nbasis) column-major,
+    YCutoff/cutPrim from QUICK's schwarzoff, o += G(dense)."""
+    announce("QUICK Fortran ABI water 6-31G*")
+    from quick_amd import _lib
+    lib = _lib.lib()
+    mol = molecule("water")
+    orc = oracle_for(mol, "6-31G*")
+    qb = orc.qb
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, _ = orc.fock2e(P, cutoff=1e-12)
+    n, ns, jb = qb.nbasis, qb.nshell, qb.jbasis
+    I = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+    D = lambda v: ctypes.byref(ctypes.c_double(float(v)))
+    ierr = ctypes.c_int(0)
+    lib.gpu_new_(ctypes.byref(ierr))
+    lib.gpu_set_device_(I(0), ctypes.byref(ierr))
+    lib.gpu_init_device_(ctypes.byref(ierr))
+    assert ierr.value == 0
+    f = ctypes.c_bool(False)
+    lib.gpu_upload_method_(I(0), ctypes.byref(f), D(1.0))
+    lib.gpu_setup_(I(len(mol.symbols)), I(n), I(10), I(1), I(0), I(2))
+    xyz = np.ascontiguousarray(qb.xyz)      # xyz(3, natom) column-major == [atom][3]
+    lib.gpu_upload_xyz_(xyz.ctypes.data_as(ctypes.c_void_p))
+    MAXPRIM = 14
+    gccoeff = np.zeros((n, MAXPRIM))        # Fortran (MAXPRIM, nbasis) == C [nbasis][MAXPRIM]
+    gcexpo = np.zeros((n, MAXPRIM))
+    for s in range(ns):
+        f0 = qb.ksumtype[s]
+        nf = sum((L + 1) * (L + 2) // 2 for L in range(qb.qstart[s], qb.qfinal[s] + 1))
+        p0, k = qb.kstart[s], qb.kprim[s]
+        for fi in range(f0, f0 + nf):
+            gcexpo[fi, :k] = qb.gcexpo[p0:p0 + k]
+        for L in range(qb.qstart[s], qb.qfinal[s] + 1):
+            gccoeff[f0 + L, :k] = qb.gccoeff[p0:p0 + k, L]
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    kstart, katom = i32(qb.kstart + 1), i32(qb.katom + 1)
+    kprim, ksum = i32(qb.kprim), i32(qb.ksumtype + 1)
+    qs, qf = i32(qb.qstart), i32(qb.qfinal)
+    dummy_i = np.zeros(max(n, ns) * 4 + 8, dtype=np.int32)
+    dummy_d = np.zeros(max(n, 1) * 4, dtype=np.float64)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.gpu_upload_basis_.restype = None
+    lib.gpu_upload_basis_(I(ns), I(jb), I(ns), I(jb), I(int(qb.kprim.max())), vp(dummy_i), vp(dummy_i),
+                          vp(dummy_d), vp(dummy_d), vp(dummy_i), vp(dummy_i), vp(dummy_i), vp(dummy_i),
+                          vp(dummy_i), vp(kstart), vp(katom), vp(dummy_i), vp(kprim), vp(dummy_i), vp(ksum),
+                          vp(dummy_i), vp(qs), vp(qf), vp(dummy_i), vp(dummy_i), vp(gccoeff), vp(dummy_d),
+                          vp(gcexpo), vp(dummy_i))
+    Y, CP = orc.schwarz()
+    Yf, CPf = np.asfortranarray(Y), np.asfortranarray(CP)
+    lib.gpu_upload_cutoff_matrix_(Yf.ctypes.data_as(ctypes.c_void_p), CPf.ctypes.data_as(ctypes.c_void_p))
+    o = np.asfortranarray(np.full((n, n), 0.25))
+    dense = np.asfortranarray(P)
+    lib.gpu_upload_calculated_(vp(o), vp(dummy_d), vp(dummy_d), dense.ctypes.data_as(ctypes.c_void_p))
+    cm = np.asfortranarray(orc.density_cutoff(P))
+    lib.gpu_upload_cutoff_(cm.ctypes.data_as(ctypes.c_void_p), D(1e-12), D(1e-13), D(1e-10), D(1e-12))
+    lib.gpu_get_cshell_eri_(ctypes.byref(f), o.ctypes.data_as(ctypes.c_void_p))
+    lib.gpu_delete_(ctypes.byref(ierr))
+    assert np.max(np.abs((o - 0.25) - G_ref)) <= FOCK_TOL
