@@ -114,9 +114,27 @@ int qf_fock_build_batch(qf_engine *const *engines, int count, const double *cons
  * with nuclear charges[natom]; computed on the GPU. */
 int qf_one_electron(qf_engine *e, const double *charges, double *S, double *T, double *V);
 
-/* Algorithmic FLOPs of the last build (per the committed class FLOP table,
- * counted on the host for the Schwarz-surviving quartets). */
+/* Algorithmic FLOPs of a build with the density of the last build: the
+ * quartets and primitive quartets that pass the screens, counted on the
+ * device, priced with the generated per-class FLOP table. */
 double qf_flops_estimate(qf_engine *e);
+
+/* Per-launch introspection: a build issues one kernel launch per
+ * (bra class, ket class, precision) that has work. */
+int qf_launch_count(qf_engine *e);
+int qf_launch_info(qf_engine *e, int idx, int *bra_class, int *ket_class, int *fp32, int *ntasks);
+int qf_launch_flops(qf_engine *e, double *flops);
+int qf_launch_counts(qf_engine *e, long long *quartets, long long *prim_quartets);
+
+/* Digestion tuning: the FP64 atomics of concurrent waves are spread over
+ * `replicas` copies of G (summed when the build finishes), and tasks inside
+ * a launch are permuted so that concurrent waves touch different kets. */
+int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
+
+/* HIP-event timing of every class launch (on the build stream); read sums
+ * the durations per launch index since the last read. */
+int qf_set_timing(qf_engine *e, int enable);
+int qf_read_timing(qf_engine *e, double *ms_per_launch, int *calls_per_launch);
 
 /* ------------------------------------------------------------------ */
 /* QUICK-compatible layer (src/gpu/hip/gpu.cu, src/gpu/gpu_get2e_getxc_drivers.h) */

@@ -8,7 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
-LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libquickfock.so")
+LIBPATH = os.environ.get("QUICKFOCK_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                           "libquickfock.so")
 
 _lib = None
 
@@ -55,6 +56,13 @@ def lib():
                                                ctypes.POINTER(vp), vp]),
         "qf_one_electron": (ctypes.c_int, [vp, dp, dp, dp, dp]),
         "qf_flops_estimate": (ctypes.c_double, [vp]),
+        "qf_launch_count": (ctypes.c_int, [vp]),
+        "qf_launch_info": (ctypes.c_int, [vp, ctypes.c_int, ip, ip, ip, ip]),
+        "qf_launch_flops": (ctypes.c_int, [vp, dp]),
+        "qf_launch_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+        "qf_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_set_tuning": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "qf_read_timing": (ctypes.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

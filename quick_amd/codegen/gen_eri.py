@@ -262,6 +262,18 @@ def quartet_classes():
     return out
 
 
+def min_waves(nacc, ni):
+    """Occupancy target per class (caps VGPRs: 8 waves -> 64, 4 -> 128, 2 -> 256)."""
+    size = nacc + ni
+    if size <= 8:
+        return 4
+    if size <= 40:
+        return 3
+    if size <= 80:
+        return 2
+    return 1
+
+
 def boys_setup_flop(L):
     """Boys function (fit polynomial, exp, sqrt, recursion) plus the per
     primitive-quartet setup (1/(z+n), PQ, T, prefactor, PA/WP/QC/WQ):
@@ -295,6 +307,8 @@ def gen_class(La, Lb, Lc, Ld):
     src.append("  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L))
     src.append("  static constexpr int NACC = %d, NI = %d;" % (nacc, ni))
     src.append("  static constexpr int VRR_FLOP = %d, HRR_FLOP = %d;" % (vflops, hflops))
+    src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets"
+               % min_waves(nacc, ni))
     src.append("  template <typename R>")
     src.append("  __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
     src += ["    " + l for l in vlines]

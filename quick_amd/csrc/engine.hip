@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -109,12 +110,16 @@ __global__ void k_zero(double* __restrict__ p, size_t count)
         p[i] = 0.0;
 }
 
-__global__ void k_finalize(const double* __restrict__ Gt, double* __restrict__ Gs, double* __restrict__ F, int n)
+__global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* __restrict__ Gs,
+                           double* __restrict__ F, int n)
 {
+    // G = sum over copies of (Gt + Gt^T); F += G
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
     const int i = idx / n, j = idx % n;
-    const double g = Gt[i * n + j] + Gt[j * n + i];
+    const size_t n2 = (size_t)n * n;
+    double g = 0.0;
+    for (int r = 0; r < replicas; ++r) g += Gt[r * n2 + i * n + j] + Gt[r * n2 + j * n + i];
     Gs[idx] = g;
     if (F) F[idx] += g;
 }
@@ -122,7 +127,7 @@ __global__ void k_finalize(const double* __restrict__ Gt, double* __restrict__ G
 // Counts, per quartet class, the quartets and primitive quartets that pass
 // the screens (same tests as the Fock kernel); used for the FLOP model.
 __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf::PairBlock* __restrict__ blocks,
-                        const qf::MolDesc* __restrict__ mols, unsigned long long* __restrict__ counts)
+                        const qf::MolDesc* __restrict__ mols, unsigned long long* __restrict__ counts, int slot)
 {
     const int tix = blockIdx.x;
     if (tix >= ntasks) return;
@@ -151,9 +156,8 @@ __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf
         for (int jp = 0; jp < kb.K; ++jp)
             if (c1 * kb.prim[(qf::PF_CUT * kb.K + jp) * kb.n + j] > md.primlim) ++np;
     }
-    const int q = bb.cls * 6 + kb.cls;
-    atomicAdd(&counts[2 * q], 1ull);
-    atomicAdd(&counts[2 * q + 1], np);
+    atomicAdd(&counts[2 * slot], 1ull);
+    atomicAdd(&counts[2 * slot + 1], np);
 }
 
 }  // namespace
@@ -174,6 +178,8 @@ struct qf_engine {
     double cutoff = 1e-12, primlim = 1e-13, hyb = 1.0;
     int prec = QF_PREC_FP64;
     double split = 1e-4;
+    int replicas = 8;        // copies of G the atomics are spread over
+    int shuffle = 1;         // permute tasks inside a launch (decorrelates atomic targets)
     // pair blocks
     std::vector<HostBlock> blocks;
     std::vector<qf::PairBlock> hblocks;
@@ -195,6 +201,15 @@ struct qf_engine {
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     qf::MolDesc mol{};
+    // per-launch timing (HIP events on the build stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> ev_launch;        // launch index of each recorded (start, stop) pair
+    ~qf_engine()
+    {
+        for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+    }
 
     int set_device() { return hipSetDevice(device) == hipSuccess ? QF_OK : fail(QF_ERR_BADDEVICE, "hipSetDevice"); }
 };
@@ -522,6 +537,17 @@ int prepare(qf_engine* e)
                     }
                 }
                 const int count = (int)e->tasks.size() - offset;
+                if (e->shuffle && count > 1) {
+                    // deterministic permutation: concurrent waves then hit different kets
+                    uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)(cb * 12 + ck * 2 + fp32);
+                    for (int q = count - 1; q > 0; --q) {
+                        st ^= st << 13;
+                        st ^= st >> 7;
+                        st ^= st << 17;
+                        const int r = (int)(st % (uint64_t)(q + 1));
+                        std::swap(e->tasks[offset + q], e->tasks[offset + r]);
+                    }
+                }
                 if (count > 0) e->launches.push_back({cb, ck, fp32, offset, count});
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
             }
@@ -529,7 +555,7 @@ int prepare(qf_engine* e)
 
     // ---- per-build buffers -------------------------------------------------
     const size_t n2 = (size_t)e->nbasis * e->nbasis;
-    QF_HIP(e->d_Gt.resize(n2));
+    QF_HIP(e->d_Gt.resize(n2 * e->replicas));
     QF_HIP(e->d_Gs.resize(n2));
     QF_HIP(e->d_P.resize(n2));
     QF_HIP(e->d_cm.resize((size_t)e->nshell * e->nshell));
@@ -543,7 +569,7 @@ int prepare(qf_engine* e)
     QF_HIP(e->d_f0.upload(f0));
     QF_HIP(e->d_f1.upload(f1));
     QF_HIP(e->d_mol.resize(1));
-    QF_HIP(e->d_counts.resize(2 * 36));
+    QF_HIP(e->d_counts.resize(2 * std::max<size_t>(1, e->launches.size())));
     e->stats.device_bytes = (long long)(e->d_info.bytes() + e->d_prim.bytes() + e->d_tasks.bytes() +
                                         e->d_Gt.bytes() * 3 + e->d_cm.bytes() + e->d_blocks.bytes());
     e->prepared = true;
@@ -569,15 +595,32 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
     md.cutoff = e->cutoff;
     md.primlim = e->primlim;
     md.hyb = e->hyb;
+    md.replicas = e->replicas;
     e->mol = md;
     QF_HIP(hipMemcpyAsync(e->d_mol.p, &e->mol, sizeof(md), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n);
-    for (const auto& L : e->launches) {
+    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n * e->replicas);
+    for (size_t li = 0; li < e->launches.size(); ++li) {
+        const auto& L = e->launches[li];
         qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
         if (!f) return fail(QF_ERR_ARG, "no kernel for class");
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (e->timing) {
+            while (e->ev_pool.size() < e->ev_used + 2) {
+                hipEvent_t ev;
+                QF_HIP(hipEventCreate(&ev));
+                e->ev_pool.push_back(ev);
+            }
+            ea = e->ev_pool[e->ev_used];
+            eb = e->ev_pool[e->ev_used + 1];
+            e->ev_used += 2;
+            e->ev_launch.push_back((int)li);
+            QF_HIP(hipEventRecord(ea, s));
+        }
         f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, L.fp32, s);
+        if (e->timing) QF_HIP(hipEventRecord(eb, s));
     }
-    hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, e->d_Gt.p, e->d_Gs.p, dF, n);
+    hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, e->d_Gt.p, e->replicas, e->d_Gs.p,
+                       dF, n);
     QF_HIP(hipGetLastError());
     return QF_OK;
 }
@@ -776,21 +819,109 @@ int qf_fock_build_batch(qf_engine* const* engines, int count, const double* cons
 
 double qf_flops_estimate(qf_engine* e)
 {
-    if (!e || !e->prepared) return -1.0;
-    // counts with the density of the last build
-    hipStream_t s = 0;
-    if (hipMemsetAsync(e->d_counts.p, 0, 2 * 36 * sizeof(unsigned long long), s) != hipSuccess) return -1.0;
-    for (const auto& L : e->launches)
-        hipLaunchKernelGGL(k_count, dim3(L.count), dim3(qf::WAVE), 0, s, e->d_tasks.p + L.offset, L.count,
-                           e->d_blocks.p, e->d_mol.p, e->d_counts.p);
-    std::vector<unsigned long long> c(72);
-    if (hipMemcpy(c.data(), e->d_counts.p, 72 * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess)
-        return -1.0;
+    const int n = qf_launch_count(e);
+    if (n < 0) return -1.0;
+    std::vector<double> f(std::max(n, 1));
+    if (qf_launch_flops(e, f.data()) != QF_OK) return -1.0;
     double tot = 0.0;
-    for (int cb = 0; cb < 6; ++cb)
-        for (int ck = 0; ck < 6; ++ck) tot += qf::class_flops(cb, ck, c[2 * (cb * 6 + ck)], c[2 * (cb * 6 + ck) + 1]);
+    for (int k = 0; k < n; ++k) tot += f[k];
     e->flops_last = tot;
     return tot;
+}
+
+int qf_launch_count(qf_engine* e)
+{
+    if (!e || !e->prepared) return -1;
+    return (int)e->launches.size();
+}
+
+int qf_launch_info(qf_engine* e, int idx, int* cb, int* ck, int* fp32, int* ntasks)
+{
+    if (!e || !e->prepared || idx < 0 || idx >= (int)e->launches.size()) return fail(QF_ERR_ARG, "launch index");
+    const auto& L = e->launches[idx];
+    *cb = L.cb;
+    *ck = L.ck;
+    *fp32 = L.fp32;
+    *ntasks = L.count;
+    return QF_OK;
+}
+
+int qf_launch_flops(qf_engine* e, double* flops)
+{
+    // Counts, with the density of the last build, the quartets and primitive
+    // quartets that pass the screens in every launch, and prices them with
+    // the generated per-class FLOP table.
+    if (!e || !e->prepared) return fail(QF_ERR_STATE, "not prepared");
+    int rc = e->set_device();
+    if (rc) return rc;
+    const size_t nl = e->launches.size();
+    QF_HIP(hipMemset(e->d_counts.p, 0, 2 * std::max<size_t>(nl, 1) * sizeof(unsigned long long)));
+    for (size_t k = 0; k < nl; ++k) {
+        const auto& L = e->launches[k];
+        hipLaunchKernelGGL(k_count, dim3(L.count), dim3(qf::WAVE), 0, 0, e->d_tasks.p + L.offset, L.count,
+                           e->d_blocks.p, e->d_mol.p, e->d_counts.p, (int)k);
+    }
+    QF_HIP(hipGetLastError());
+    std::vector<unsigned long long> c(2 * std::max<size_t>(nl, 1));
+    QF_HIP(hipMemcpy(c.data(), e->d_counts.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nl; ++k) flops[k] = qf::class_flops(e->launches[k].cb, e->launches[k].ck, c[2 * k], c[2 * k + 1]);
+    return QF_OK;
+}
+
+int qf_launch_counts(qf_engine* e, long long* quartets, long long* prims)
+{
+    if (!e || !e->prepared) return fail(QF_ERR_STATE, "not prepared");
+    const size_t nl = e->launches.size();
+    std::vector<double> tmp(std::max<size_t>(nl, 1));
+    int rc = qf_launch_flops(e, tmp.data());
+    if (rc) return rc;
+    std::vector<unsigned long long> c(2 * std::max<size_t>(nl, 1));
+    QF_HIP(hipMemcpy(c.data(), e->d_counts.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nl; ++k) {
+        quartets[k] = (long long)c[2 * k];
+        prims[k] = (long long)c[2 * k + 1];
+    }
+    return QF_OK;
+}
+
+int qf_set_tuning(qf_engine* e, int replicas, int shuffle)
+{
+    if (!e || replicas < 1 || replicas > 256) return fail(QF_ERR_ARG, "replicas must be in 1..256");
+    e->replicas = replicas;
+    e->shuffle = shuffle != 0;
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_timing(qf_engine* e, int enable)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    e->timing = enable != 0;
+    e->ev_used = 0;
+    e->ev_launch.clear();
+    return QF_OK;
+}
+
+int qf_read_timing(qf_engine* e, double* ms_per_launch, int* calls_per_launch)
+{
+    // Sums the event-measured duration of every recorded class launch since
+    // the last read, per launch index; waits for the last recorded event.
+    if (!e || !e->prepared) return fail(QF_ERR_STATE, "not prepared");
+    const size_t nl = e->launches.size();
+    for (size_t k = 0; k < nl; ++k) {
+        ms_per_launch[k] = 0.0;
+        calls_per_launch[k] = 0;
+    }
+    if (e->ev_used) QF_HIP(hipEventSynchronize(e->ev_pool[e->ev_used - 1]));
+    for (size_t r = 0; r < e->ev_launch.size(); ++r) {
+        float ms = 0.0f;
+        QF_HIP(hipEventElapsedTime(&ms, e->ev_pool[2 * r], e->ev_pool[2 * r + 1]));
+        ms_per_launch[e->ev_launch[r]] += ms;
+        calls_per_launch[e->ev_launch[r]] += 1;
+    }
+    e->ev_used = 0;
+    e->ev_launch.clear();
+    return QF_OK;
 }
 
 }  // extern "C"

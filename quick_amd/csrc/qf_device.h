@@ -9,6 +9,34 @@
 
 namespace qf {
 
+// Pointers fetched from descriptors are generic; these casts tell the
+// compiler which memory they address, so wave-uniform reads become scalar
+// loads (constant address space) and per-lane reads / atomics use the
+// global-memory instructions instead of flat ones.  (The host pass of the
+// HIP compiler sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QF_AS_CONST __attribute__((address_space(4)))
+#define QF_AS_GLOBAL __attribute__((address_space(1)))
+#else
+#define QF_AS_CONST
+#define QF_AS_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ const QF_AS_CONST T* cptr(const T* p)
+{
+    return (const QF_AS_CONST T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ const QF_AS_GLOBAL T* gptr(const T* p)
+{
+    return (const QF_AS_GLOBAL T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ QF_AS_GLOBAL T* gptr_rw(T* p)
+{
+    return (QF_AS_GLOBAL T*)(p);
+}
+
 template <typename R>
 struct VrrPar {
     R PA[3], WP[3], QC[3], WQ[3];
@@ -76,12 +104,15 @@ __device__ __forceinline__ void boys_quick(const double X, const int maxm, doubl
 #pragma unroll
         for (int m = 1; m <= L; ++m) F[m] = ((2 * m - 1) * F[m - 1] - E) * h;
     } else {
+        // maxm exceeds L only by the s parts of SP shells (at most 4)
         const double twot = X * 2.0;
         double f = WW1;
-        for (int m = maxm - 1; m >= L; --m) f = (twot * f + E) / (double)(m * 2 + 1);
+#pragma unroll
+        for (int m = L + 3; m >= L; --m)
+            if (m < maxm) f = (twot * f + E) * (1.0 / (2 * m + 1));
         F[L] = f;
 #pragma unroll
-        for (int m = L - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) / (double)(m * 2 + 1);
+        for (int m = L - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) * (1.0 / (2 * m + 1));
     }
 }
 
@@ -100,46 +131,67 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(QF_AS_GLOBAL double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ---------------------------------------------------------------------------
 // Contracted [e0|f0] for one (bra pair, ket pair): primitive loops with
-// QUICK's primitive screen dnmax*cutprim(ij)*cutprim(kl) > primLimit
+// QUICK's primitive screen (dnmax*cutprim(ij))*cutprim(kl) > primLimit
 // (eri.fh:270,288-289).  Primitive pairs are sorted by cutprim descending,
-// so the first failing ket primitive ends the inner loop and a failing
-// first ket primitive ends the outer loop: the same set QUICK skips.
+// so a failing bra primitive ends the inner loop and a failing first bra
+// primitive ends the outer loop: the same set QUICK skips.
+//
+// The ket primitive (different per lane) is the outer loop: its seven
+// fields are vector-loaded once and the next one is prefetched while the
+// current one is used; the bra primitives (the same for the whole wave) are
+// the inner loop and come through scalar loads.
 // ---------------------------------------------------------------------------
-template <class C, typename R>
+template <class C, typename R, bool BRA_UNIFORM>
 __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
                                          int j, const PairInfo& ki, double dn, double primlim, bool screen,
                                          int maxm, R (&acc)[C::NACC])
 {
-    const double* __restrict__ bp = bb.prim;
-    const double* __restrict__ kp = kb.prim;
+    const auto bp = BRA_UNIFORM ? (const double*)nullptr : bb.prim;
+    const auto bpc = cptr(bb.prim);           // wave-uniform bra data
+    const auto kp = gptr(kb.prim);            // per-lane ket data
     const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
-    const double kcut0 = kp[(PF_CUT * kK) * kn + j];
-    for (int ip = 0; ip < bK; ++ip) {
-        const double zb = bp[(PF_ZETA * bK + ip) * bn + i];
-        const double ozb = bp[(PF_OOZ2 * bK + ip) * bn + i];
-        const double Pb[3] = {bp[(PF_PX * bK + ip) * bn + i], bp[(PF_PY * bK + ip) * bn + i],
-                              bp[(PF_PZ * bK + ip) * bn + i]};
-        const double Kb = bp[(PF_KC * bK + ip) * bn + i];
-        const double c1 = dn * bp[(PF_CUT * bK + ip) * bn + i];
-        if (screen && !(c1 * kcut0 > primlim)) break;
-        for (int jp = 0; jp < kK; ++jp) {
-            if (screen && !(c1 * kp[(PF_CUT * kK + jp) * kn + j] > primlim)) break;
-            const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
-            const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
-            const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
-                                  kp[(PF_PZ * kK + jp) * kn + j]};
-            const double Kk = kp[(PF_KC * kK + jp) * kn + j];
-            const double inv = 1.0 / (zb + zk);
+#define BP(idx) (BRA_UNIFORM ? bpc[idx] : bp[idx])
+    const double c0 = dn * BP((PF_CUT * bK) * bn + i);     // largest bra cutprim
+    double nz = kp[(PF_ZETA * kK) * kn + j], noz = kp[(PF_OOZ2 * kK) * kn + j];
+    double nqx = kp[(PF_PX * kK) * kn + j], nqy = kp[(PF_PY * kK) * kn + j], nqz = kp[(PF_PZ * kK) * kn + j];
+    double nk = kp[(PF_KC * kK) * kn + j], nc = kp[(PF_CUT * kK) * kn + j];
+    for (int jp = 0; jp < kK; ++jp) {
+        const double zk = nz, ozk = noz, Kk = nk, cutk = nc;
+        const double Qk[3] = {nqx, nqy, nqz};
+        if (jp + 1 < kK) {
+            const int o = jp + 1;
+            nz = kp[(PF_ZETA * kK + o) * kn + j];
+            noz = kp[(PF_OOZ2 * kK + o) * kn + j];
+            nqx = kp[(PF_PX * kK + o) * kn + j];
+            nqy = kp[(PF_PY * kK + o) * kn + j];
+            nqz = kp[(PF_PZ * kK + o) * kn + j];
+            nk = kp[(PF_KC * kK + o) * kn + j];
+            nc = kp[(PF_CUT * kK + o) * kn + j];
+        }
+        if (screen && !(c0 * cutk > primlim)) break;
+        const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
+        for (int ip = 0; ip < bK; ++ip) {
+            if (screen && !((dn * BP((PF_CUT * bK + ip) * bn + i)) * cutk > primlim)) break;
+            const double zb = BP((PF_ZETA * bK + ip) * bn + i);
+            const double ozb = BP((PF_OOZ2 * bK + ip) * bn + i);
+            const double Pb[3] = {BP((PF_PX * bK + ip) * bn + i), BP((PF_PY * bK + ip) * bn + i),
+                                  BP((PF_PZ * bK + ip) * bn + i)};
+            const double Kb = BP((PF_KC * bK + ip) * bn + i);
+            const double rs = rsqrt(zb + zk);          // 1/sqrt(zeta+eta)
+            const double inv = rs * rs;
             const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
             const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
             const double T = rpq * (zb * zk * inv);
             double F[C::L + 1];
             boys_quick<C::L>(T, maxm, F);
-            const double pref = Kb * Kk * sqrt(inv);
+            const double pref = Kb * Kk * rs;
             R Fr[C::L + 1];
 #pragma unroll
             for (int m = 0; m <= C::L; ++m) Fr[m] = R(F[m] * pref);
@@ -149,7 +201,7 @@ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairI
             for (int d = 0; d < 3; ++d) {
                 p.PA[d] = R(Pb[d] - bi.A[d]);
                 p.WP[d] = R(-wp * PQ[d]);
-                p.QC[d] = R(Qk[d] - ki.A[d]);
+                p.QC[d] = R(QC[d]);
                 p.WQ[d] = R(wq * PQ[d]);
             }
             p.oo2z = R(ozb);
@@ -160,6 +212,7 @@ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairI
             C::template vrr<R>(Fr, p, acc);
         }
     }
+#undef BP
 }
 
 // ---------------------------------------------------------------------------
@@ -173,25 +226,27 @@ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairI
 // before its atomics; the other five go straight to FP64 atomics.
 // ---------------------------------------------------------------------------
 template <class C, typename R>
-__global__ __launch_bounds__(WAVE) void fock_kernel(const Task* __restrict__ tasks,
+__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
+void fock_kernel(const Task* __restrict__ tasks,
                                                     const PairBlock* __restrict__ blocks,
                                                     const MolDesc* __restrict__ mols)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb), nc = ncart(C::Lc), nd = ncart(C::Ld);
-    const Task t = tasks[blockIdx.x];
-    const PairBlock bb = blocks[t.bra_block];
-    const PairBlock kb = blocks[t.ket_block];
-    const MolDesc md = mols[bb.mol];
+    const Task t = cptr(tasks)[blockIdx.x];
+    const PairBlock bb = cptr(blocks)[t.bra_block];
+    const PairBlock kb = cptr(blocks)[t.ket_block];
+    const MolDesc md = cptr(mols)[bb.mol];
     const int lane = threadIdx.x;
     const int i = t.bra_idx;
-    const PairInfo bi = bb.info[i];
+    const PairInfo bi = cptr(bb.info)[i];
     bool active = lane < t.ket_count;
     const int j = active ? t.ket_start + lane : t.ket_start;
-    const PairInfo ki = kb.info[j];
+    const PairInfo ki = gptr(kb.info)[j];
 
     const int ns = md.nshell;
-    const double* __restrict__ cm = md.cutmat;
-    double dn = 4.0 * cm[bi.sa * ns + bi.sb];
+    const auto cmc = cptr(md.cutmat);
+    const auto cm = gptr(md.cutmat);
+    double dn = 4.0 * cmc[bi.sa * ns + bi.sb];
     dn = fmax(dn, 4.0 * cm[ki.sa * ns + ki.sb]);
     dn = fmax(dn, cm[bi.sa * ns + ki.sb]);
     dn = fmax(dn, cm[bi.sa * ns + ki.sa]);
@@ -203,7 +258,7 @@ __global__ __launch_bounds__(WAVE) void fock_kernel(const Task* __restrict__ tas
     R acc[C::NACC];
 #pragma unroll
     for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
-    if (active) contract<C, R>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
+    if (active) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
 
     R I[C::NI];
     {
@@ -213,8 +268,10 @@ __global__ __launch_bounds__(WAVE) void fock_kernel(const Task* __restrict__ tas
     }
 
     const int n = md.nbasis;
-    const double* __restrict__ P = md.P;
-    double* __restrict__ G = md.G;
+    const auto Pc = cptr(md.P);      // bra block: same for every lane
+    const auto P = gptr(md.P);
+    // spread the atomics of concurrent waves over several copies of G
+    const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * n * n);
     const double s = active ? 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) *
                                   (bi.id != ki.id ? 2.0 : 1.0)
                             : 0.0;
@@ -224,7 +281,7 @@ __global__ __launch_bounds__(WAVE) void fock_kernel(const Task* __restrict__ tas
 #pragma unroll
     for (int a = 0; a < na; ++a)
 #pragma unroll
-        for (int b = 0; b < nb; ++b) Dab[a * nb + b] = R(P[(bi.fa + a) * n + bi.fb + b]);
+        for (int b = 0; b < nb; ++b) Dab[a * nb + b] = R(Pc[(bi.fa + a) * n + bi.fb + b]);
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
@@ -321,7 +378,7 @@ __global__ __launch_bounds__(WAVE) void schwarz_kernel(PairBlock blk, double* __
     double acc[C::NACC];
 #pragma unroll
     for (int k = 0; k < C::NACC; ++k) acc[k] = 0.0;
-    contract<C, double>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc);
+    contract<C, double, false>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc);
     double I[C::NI];
     const double AB[3] = {pi.AB[0], pi.AB[1], pi.AB[2]};
     C::template hrr<double>(acc, AB, AB, I);

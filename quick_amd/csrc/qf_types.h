@@ -38,6 +38,8 @@ struct MolDesc {
     const double* cutmat;   // nshell x nshell max |P| per original shell pair
     int nbasis, nshell;
     double cutoff, primlim, hyb;
+    int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
+    int pad;
 };
 
 struct Task {

@@ -117,5 +117,45 @@ class FockEngine:
         return self.lib.qf_device_g(self.h)
 
     def flops(self):
-        """Algorithmic FLOPs of the last build (committed per-class table)."""
+        """Algorithmic FLOPs of a build with the last density (per-class table)."""
         return self.lib.qf_flops_estimate(self.h)
+
+    CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd"]
+
+    def launches(self):
+        """[(name, fp32, ntasks)] of the class launches a build issues."""
+        self.prepare()
+        out = []
+        for k in range(self.lib.qf_launch_count(self.h)):
+            cb, ck, f32, nt = (ctypes.c_int() for _ in range(4))
+            L.check(self.lib.qf_launch_info(self.h, k, ctypes.byref(cb), ctypes.byref(ck), ctypes.byref(f32),
+                                            ctypes.byref(nt)), "qf_launch_info")
+            out.append(("(%s|%s)" % (self.CLASS_NAMES[cb.value], self.CLASS_NAMES[ck.value]), bool(f32.value),
+                        nt.value))
+        return out
+
+    def launch_flops(self):
+        n = self.lib.qf_launch_count(self.h)
+        f = np.zeros(max(n, 1))
+        L.check(self.lib.qf_launch_flops(self.h, L.dptr(f)), "qf_launch_flops")
+        return f[:n]
+
+    def launch_counts(self):
+        n = self.lib.qf_launch_count(self.h)
+        q = (ctypes.c_longlong * max(n, 1))()
+        p = (ctypes.c_longlong * max(n, 1))()
+        L.check(self.lib.qf_launch_counts(self.h, q, p), "qf_launch_counts")
+        return np.array(q[:n]), np.array(p[:n])
+
+    def set_tuning(self, replicas=8, shuffle=True):
+        L.check(self.lib.qf_set_tuning(self.h, int(replicas), 1 if shuffle else 0), "qf_set_tuning")
+
+    def set_timing(self, on=True):
+        L.check(self.lib.qf_set_timing(self.h, 1 if on else 0), "qf_set_timing")
+
+    def read_timing(self):
+        n = self.lib.qf_launch_count(self.h)
+        ms = np.zeros(max(n, 1))
+        calls = np.zeros(max(n, 1), dtype=np.int32)
+        L.check(self.lib.qf_read_timing(self.h, L.dptr(ms), L.iptr(calls)), "qf_read_timing")
+        return ms[:n], calls[:n]
