@@ -310,11 +310,11 @@ def gen_class(La, Lb, Lc, Ld):
     src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets"
                % min_waves(nacc, ni))
     src.append("  template <typename R>")
-    src.append("  __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
+    src.append("  __host__ __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
     src += ["    " + l for l in vlines]
     src.append("  }")
     src.append("  template <typename R>")
-    src.append("  __device__ static __forceinline__ void hrr(const R (&acc)[NACC], const R (&AB)[3], const R (&CD)[3], R (&I)[NI]) {")
+    src.append("  __host__ __device__ static __forceinline__ void hrr(const R (&acc)[NACC], const R (&AB)[3], const R (&CD)[3], R (&I)[NI]) {")
     src += ["    " + l for l in hlines]
     src.append("  }")
     src.append("};")

@@ -83,6 +83,8 @@ struct QShell {
 struct HostBlock {
     int cls = 0, K = 0;
     std::vector<qf::PairInfo> info;
+    std::vector<int> bperm;                      // pair indices grouped by first Q-shell
+    std::vector<std::pair<int, int>> groups;     // [start, end) in bperm, one per first Q-shell
     std::vector<double> prim;   // PF_N * K * n, filled at finalisation
     std::vector<std::vector<double>> rows;   // per pair: PF_N*K values (field-major)
 };
@@ -127,7 +129,8 @@ __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* 
 // Counts, per quartet class, the quartets and primitive quartets that pass
 // the screens (same tests as the Fock kernel); used for the FLOP model.
 __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf::PairBlock* __restrict__ blocks,
-                        const qf::MolDesc* __restrict__ mols, unsigned long long* __restrict__ counts, int slot)
+                        const qf::MolDesc* __restrict__ mols, unsigned long long* __restrict__ counts, int slot,
+                        int fp32)
 {
     const int tix = blockIdx.x;
     if (tix >= ntasks) return;
@@ -137,26 +140,32 @@ __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf
     const qf::MolDesc md = mols[bb.mol];
     const int lane = threadIdx.x;
     if (lane >= t.ket_count) return;
-    const int i = t.bra_idx, j = t.ket_start + lane;
-    const qf::PairInfo bi = bb.info[i];
+    const int j = t.ket_start + lane;
     const qf::PairInfo ki = kb.info[j];
     const int ns = md.nshell;
     const double* cm = md.cutmat;
-    double dn = 4.0 * cm[bi.sa * ns + bi.sb];
-    dn = fmax(dn, 4.0 * cm[ki.sa * ns + ki.sb]);
-    dn = fmax(dn, cm[bi.sa * ns + ki.sb]);
-    dn = fmax(dn, cm[bi.sa * ns + ki.sa]);
-    dn = fmax(dn, cm[bi.sb * ns + ki.sa]);
-    dn = fmax(dn, cm[bi.sb * ns + ki.sb]);
-    const double y = bi.ycut * ki.ycut;
-    if (!(y > md.cutoff && y * dn > md.cutoff)) return;
-    unsigned long long np = 0;
-    for (int ip = 0; ip < bb.K; ++ip) {
-        const double c1 = dn * bb.prim[(qf::PF_CUT * bb.K + ip) * bb.n + i];
-        for (int jp = 0; jp < kb.K; ++jp)
-            if (c1 * kb.prim[(qf::PF_CUT * kb.K + jp) * kb.n + j] > md.primlim) ++np;
+    unsigned long long nq = 0, np = 0;
+    for (int kk = t.kb; kk < t.ke; ++kk) {
+        const int i = bb.bperm[kk];
+        const qf::PairInfo bi = bb.info[i];
+        double dn = 4.0 * cm[bi.sa * ns + bi.sb];
+        dn = fmax(dn, 4.0 * cm[ki.sa * ns + ki.sb]);
+        dn = fmax(dn, cm[bi.sa * ns + ki.sb]);
+        dn = fmax(dn, cm[bi.sa * ns + ki.sa]);
+        dn = fmax(dn, cm[bi.sb * ns + ki.sa]);
+        dn = fmax(dn, cm[bi.sb * ns + ki.sb]);
+        const double y = bi.ycut * ki.ycut;
+        if (!(y > md.cutoff && y * dn > md.cutoff)) continue;
+        if (t.same_block && j > i) continue;
+        if (fp32 ? !(y < md.split) : !(t.all_fp64 || y >= md.split)) continue;
+        ++nq;
+        for (int ip = 0; ip < bb.K; ++ip) {
+            const double c1 = dn * bb.prim[(qf::PF_CUT * bb.K + ip) * bb.n + i];
+            for (int jp = 0; jp < kb.K; ++jp)
+                if (c1 * kb.prim[(qf::PF_CUT * kb.K + jp) * kb.n + j] > md.primlim) ++np;
+        }
     }
-    atomicAdd(&counts[2 * slot], 1ull);
+    atomicAdd(&counts[2 * slot], nq);
     atomicAdd(&counts[2 * slot + 1], np);
 }
 
@@ -178,6 +187,7 @@ struct qf_engine {
     double cutoff = 1e-12, primlim = 1e-13, hyb = 1.0;
     int prec = QF_PREC_FP64;
     double split = 1e-4;
+    double split_thr = 0.0;  // Ycut_ij*Ycut_kl below which quartets run in FP32
     int replicas = 8;        // copies of G the atomics are spread over
     int shuffle = 1;         // permute tasks inside a launch (decorrelates atomic targets)
     // pair blocks
@@ -194,6 +204,7 @@ struct qf_engine {
     // device
     DevBuf<qf::PairInfo> d_info;
     DevBuf<double> d_prim;
+    DevBuf<int> d_perm;
     DevBuf<qf::PairBlock> d_blocks;
     DevBuf<qf::MolDesc> d_mol;
     DevBuf<qf::Task> d_tasks;
@@ -294,7 +305,7 @@ void make_pair(const qf_engine* e, int qa, int qb, int ip, int jp, bool use_cut,
 
 // Assemble host blocks into flat device arrays; fills hblocks (device pointers).
 int upload_blocks(qf_engine* e, std::vector<HostBlock>& blocks, DevBuf<qf::PairInfo>& d_info,
-                  DevBuf<double>& d_prim, std::vector<qf::PairBlock>& out, int mol)
+                  DevBuf<double>& d_prim, std::vector<qf::PairBlock>& out, int mol, DevBuf<int>* d_perm = nullptr)
 {
     size_t ninfo = 0, nprimv = 0;
     for (auto& b : blocks) {
@@ -318,11 +329,19 @@ int upload_blocks(qf_engine* e, std::vector<HostBlock>& blocks, DevBuf<qf::PairI
     }
     QF_HIP(d_info.upload(info));
     QF_HIP(d_prim.upload(prim));
+    std::vector<int> perm;
+    std::vector<size_t> permoff;
+    for (auto& b : blocks) {
+        permoff.push_back(perm.size());
+        perm.insert(perm.end(), b.bperm.begin(), b.bperm.end());
+    }
+    if (d_perm) QF_HIP(d_perm->upload(perm));
     out.clear();
     for (size_t k = 0; k < blocks.size(); ++k) {
         qf::PairBlock pb;
         pb.info = d_info.p + ioff[k];
         pb.prim = d_prim.p + poff[k];
+        pb.bperm = (d_perm && !blocks[k].bperm.empty()) ? d_perm->p + permoff[k] : nullptr;
         pb.n = (int)blocks[k].info.size();
         pb.K = blocks[k].K;
         pb.cls = blocks[k].cls;
@@ -491,13 +510,29 @@ int prepare(qf_engine* e)
         b.info.swap(info2);
         b.rows.swap(rows2);
     }
-    rc = upload_blocks(e, e->blocks, e->d_info, e->d_prim, e->hblocks, 0);
+    for (auto& b : e->blocks) {
+        // bra order: grouped by first Q-shell a, Schwarz-descending inside a group
+        const int nb = (int)b.info.size();
+        b.bperm.resize(nb);
+        std::iota(b.bperm.begin(), b.bperm.end(), 0);
+        std::stable_sort(b.bperm.begin(), b.bperm.end(),
+                         [&](int x, int y) { return b.info[x].qa < b.info[y].qa; });
+        b.groups.clear();
+        for (int k = 0; k < nb;) {
+            int e2 = k;
+            while (e2 < nb && b.info[b.bperm[e2]].qa == b.info[b.bperm[k]].qa) ++e2;
+            b.groups.push_back({k, e2});
+            k = e2;
+        }
+    }
+    rc = upload_blocks(e, e->blocks, e->d_info, e->d_prim, e->hblocks, 0, &e->d_perm);
     if (rc) return rc;
     for (auto& b : e->blocks) b.rows.clear();
     QF_HIP(e->d_blocks.upload(e->hblocks));
 
     // ---- wave tasks per quartet class and precision ------------------------
     const double thr_hi = e->split * ymax * ymax;
+    e->split_thr = thr_hi;
     e->tasks.clear();
     e->launches.clear();
     e->stats = qf_stats{};
@@ -505,7 +540,12 @@ int prepare(qf_engine* e)
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32) {
-                if (fp32 && e->prec != QF_PREC_MIXED) continue;
+                // (dd|dp) and (dd|dd) stay FP64 in mixed mode: their float
+                // kernels spill several KB per lane (little to gain) and the
+                // (dd|dp) one returned non-finite values on gfx950.
+                const bool split_class = e->prec == QF_PREC_MIXED &&
+                                         kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
+                if (fp32 && !split_class) continue;
                 const int offset = (int)e->tasks.size();
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {
                     const HostBlock& B = e->blocks[bbk];
@@ -514,23 +554,51 @@ int prepare(qf_engine* e)
                         const HostBlock& Kb = e->blocks[kbk];
                         if (Kb.cls != ck) continue;
                         if (cb == ck && kbk > bbk) continue;
-                        for (int i = 0; i < (int)B.info.size(); ++i) {
-                            const double yi = B.info[i].ycut;
-                            int n = prefix_count(Kb.info, yi, e->cutoff, true);
-                            if (cb == ck && kbk == bbk) n = std::min(n, i + 1);
-                            int lo = 0, hi = n;
-                            if (e->prec == QF_PREC_MIXED) {
-                                const int n64 = std::min(n, prefix_count(Kb.info, yi, thr_hi, false));
-                                if (fp32) lo = n64; else hi = n64;
+                        const bool same = (cb == ck && kbk == bbk);
+                        // bras per task: long loops for cheap classes (more register
+                        // accumulation, fewer atomics), short ones for expensive classes
+                        // (enough waves to fill 256 CUs)
+                        const int ni = ncart(kClassL[cb][0]) * ncart(kClassL[cb][1]) * ncart(kClassL[ck][0]) *
+                                       ncart(kClassL[ck][1]);
+                        const int mmax = ni <= 10 ? 16 : ni <= 36 ? 8 : ni <= 108 ? 4 : ni <= 324 ? 2 : 1;
+                        std::vector<std::pair<int, int>> chunks;
+                        for (const auto& g : B.groups)
+                            for (int c0 = g.first; c0 < g.second; c0 += mmax)
+                                chunks.push_back({c0, std::min(g.second, c0 + mmax)});
+                        for (const auto& g : chunks) {
+                            // per bra of the group: ket range [lo, hi) of this precision
+                            const int M = g.second - g.first;
+                            std::vector<int> lo(M), hi(M);
+                            int nmax = 0;
+                            for (int k = 0; k < M; ++k) {
+                                const int i = B.bperm[g.first + k];
+                                const double yi = B.info[i].ycut;
+                                int n = prefix_count(Kb.info, yi, e->cutoff, true);
+                                if (same) n = std::min(n, i + 1);
+                                if (!fp32) e->stats.n_quartets_bound += n;
+                                int n64 = n;
+                                if (split_class) n64 = std::min(n, prefix_count(Kb.info, yi, thr_hi, false));
+                                lo[k] = fp32 ? n64 : 0;
+                                hi[k] = fp32 ? n : n64;
+                                nmax = std::max(nmax, hi[k]);
                             }
-                            if (!fp32) e->stats.n_quartets_bound += n;
-                            for (int s = lo; s < hi; s += qf::WAVE) {
+                            for (int s0 = 0; s0 < nmax; s0 += qf::WAVE) {
+                                int kb = -1, ke = -1;
+                                for (int k = 0; k < M; ++k)
+                                    if (lo[k] < hi[k] && lo[k] < s0 + qf::WAVE && hi[k] > s0) {
+                                        if (kb < 0) kb = k;
+                                        ke = k + 1;
+                                    }
+                                if (kb < 0) continue;
                                 qf::Task t{};
                                 t.bra_block = (int)bbk;
-                                t.bra_idx = i;
+                                t.kb = g.first + kb;
+                                t.ke = g.first + ke;
                                 t.ket_block = (int)kbk;
-                                t.ket_start = s;
-                                t.ket_count = std::min(qf::WAVE, hi - s);
+                                t.ket_start = s0;
+                                t.ket_count = std::min(qf::WAVE, nmax - s0);
+                                t.same_block = same ? 1 : 0;
+                                t.all_fp64 = split_class ? 0 : 1;
                                 e->tasks.push_back(t);
                             }
                         }
@@ -538,7 +606,8 @@ int prepare(qf_engine* e)
                 }
                 const int count = (int)e->tasks.size() - offset;
                 if (e->shuffle && count > 1) {
-                    // deterministic permutation: concurrent waves then hit different kets
+                    // deterministic permutation (concurrent waves then hit different
+                    // kets), then longest tasks first so the launch ends evenly
                     uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)(cb * 12 + ck * 2 + fp32);
                     for (int q = count - 1; q > 0; --q) {
                         st ^= st << 13;
@@ -547,6 +616,9 @@ int prepare(qf_engine* e)
                         const int r = (int)(st % (uint64_t)(q + 1));
                         std::swap(e->tasks[offset + q], e->tasks[offset + r]);
                     }
+                    std::stable_sort(e->tasks.begin() + offset, e->tasks.end(), [](const qf::Task& x, const qf::Task& y) {
+                        return (x.ke - x.kb) * x.ket_count > (y.ke - y.kb) * y.ket_count;
+                    });
                 }
                 if (count > 0) e->launches.push_back({cb, ck, fp32, offset, count});
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
@@ -596,6 +668,7 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
     md.primlim = e->primlim;
     md.hyb = e->hyb;
     md.replicas = e->replicas;
+    md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
     e->mol = md;
     QF_HIP(hipMemcpyAsync(e->d_mol.p, &e->mol, sizeof(md), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n * e->replicas);
@@ -859,7 +932,7 @@ int qf_launch_flops(qf_engine* e, double* flops)
     for (size_t k = 0; k < nl; ++k) {
         const auto& L = e->launches[k];
         hipLaunchKernelGGL(k_count, dim3(L.count), dim3(qf::WAVE), 0, 0, e->d_tasks.p + L.offset, L.count,
-                           e->d_blocks.p, e->d_mol.p, e->d_counts.p, (int)k);
+                           e->d_blocks.p, e->d_mol.p, e->d_counts.p, (int)k, L.fp32);
     }
     QF_HIP(hipGetLastError());
     std::vector<unsigned long long> c(2 * std::max<size_t>(nl, 1));

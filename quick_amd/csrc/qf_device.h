@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "qf_types.h"
 
 namespace qf {
@@ -22,12 +24,12 @@ namespace qf {
 #define QF_AS_GLOBAL
 #endif
 template <typename T>
-__device__ __forceinline__ const QF_AS_CONST T* cptr(const T* p)
+__host__ __device__ __forceinline__ const QF_AS_CONST T* cptr(const T* p)
 {
     return (const QF_AS_CONST T*)(p);
 }
 template <typename T>
-__device__ __forceinline__ const QF_AS_GLOBAL T* gptr(const T* p)
+__host__ __device__ __forceinline__ const QF_AS_GLOBAL T* gptr(const T* p)
 {
     return (const QF_AS_GLOBAL T*)(p);
 }
@@ -35,6 +37,17 @@ template <typename T>
 __device__ __forceinline__ QF_AS_GLOBAL T* gptr_rw(T* p)
 {
     return (QF_AS_GLOBAL T*)(p);
+}
+
+// 1/sqrt(x): the hardware reciprocal square root on the device, the C
+// library on the host (the host path only serves CPU checks of this code).
+__host__ __device__ __forceinline__ double qf_rsqrt(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrt(x);
+#else
+    return 1.0 / std::sqrt(x);
+#endif
 }
 
 template <typename R>
@@ -53,7 +66,7 @@ struct VrrPar {
 // The fit coefficients are QUICK's; parity with QUICK needs them.
 // ---------------------------------------------------------------------------
 template <int L>
-__device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1])
+__host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1])
 {
     const double XINV = 1.0 / X;
     const double E = exp(-X);
@@ -117,7 +130,7 @@ __device__ __forceinline__ void boys_quick(const double X, const int maxm, doubl
 }
 
 // cons factors of QUICK's Cartesian components (basis.f90:567-640)
-__device__ constexpr double kCons[3][6] = {
+__host__ __device__ constexpr double kCons[3][6] = {
     {1.0, 0, 0, 0, 0, 0},
     {1.0, 1.0, 1.0, 0, 0, 0},
     {1.0, 1.7320508075688772935, 1.0, 1.7320508075688772935, 1.7320508075688772935, 1.0}};
@@ -149,7 +162,7 @@ __device__ __forceinline__ void atomic_add(QF_AS_GLOBAL double* p, double v)
 // the inner loop and come through scalar loads.
 // ---------------------------------------------------------------------------
 template <class C, typename R, bool BRA_UNIFORM>
-__device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
+__host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
                                          int j, const PairInfo& ki, double dn, double primlim, bool screen,
                                          int maxm, R (&acc)[C::NACC])
 {
@@ -184,7 +197,7 @@ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairI
             const double Pb[3] = {BP((PF_PX * bK + ip) * bn + i), BP((PF_PY * bK + ip) * bn + i),
                                   BP((PF_PZ * bK + ip) * bn + i)};
             const double Kb = BP((PF_KC * bK + ip) * bn + i);
-            const double rs = rsqrt(zb + zk);          // 1/sqrt(zeta+eta)
+            const double rs = qf_rsqrt(zb + zk);       // 1/sqrt(zeta+eta)
             const double inv = rs * rs;
             const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
             const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
@@ -216,149 +229,162 @@ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairI
 }
 
 // ---------------------------------------------------------------------------
-// Fock kernel: one wave per task = one bra pair x up to 64 ket pairs of one
-// block.  Screening per lane as getCshellEri (eri.fh:173-189).  Digestion of
-// the canonical quartet with degeneracy weight s = f/4,
-// f = (a!=b ? 2:1)(c!=d ? 2:1)(ab!=cd ? 2:1):
+// Fock kernel.  One wave per task: lanes hold up to 64 kets (c,d) of one
+// ket block; the wave loops over bras (a,b_k) of one bra block that all
+// share the first Q-shell a.  Screening per quartet as getCshellEri
+// (eri.fh:173-189).  Digestion of the canonical quartet with degeneracy
+// weight s = f/4, f = (a!=b ? 2:1)(c!=d ? 2:1)(ab!=cd ? 2:1):
 //   Gt_ab += s D_cd I, Gt_cd += s D_ab I,
-//   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc, Gt_bc -= .. D_ad, Gt_bd -= .. D_ac,
-// and G = Gt + Gt^T afterwards.  The bra block Gt_ab is summed over the wave
-// before its atomics; the other five go straight to FP64 atomics.
+//   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc I, Gt_bc -= .. D_ad I, Gt_bd -= .. D_ac I,
+// and G = Gt + Gt^T afterwards.  Because a is fixed for the wave, J_cd, K_ac
+// and K_ad stay in registers across the bra loop; J_ab is summed over the
+// wave per bra; only K_bc and K_bd go to FP64 atomics per quartet.
 // ---------------------------------------------------------------------------
 template <class C, typename R>
 __global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
-void fock_kernel(const Task* __restrict__ tasks,
-                                                    const PairBlock* __restrict__ blocks,
-                                                    const MolDesc* __restrict__ mols)
+void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ blocks,
+                 const MolDesc* __restrict__ mols)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb), nc = ncart(C::Lc), nd = ncart(C::Ld);
+    constexpr bool kFP32 = sizeof(R) == 4;
     const Task t = cptr(tasks)[blockIdx.x];
     const PairBlock bb = cptr(blocks)[t.bra_block];
     const PairBlock kb = cptr(blocks)[t.ket_block];
     const MolDesc md = cptr(mols)[bb.mol];
     const int lane = threadIdx.x;
-    const int i = t.bra_idx;
-    const PairInfo bi = cptr(bb.info)[i];
-    bool active = lane < t.ket_count;
-    const int j = active ? t.ket_start + lane : t.ket_start;
+    const bool in_range = lane < t.ket_count;
+    const int j = in_range ? t.ket_start + lane : t.ket_start;
     const PairInfo ki = gptr(kb.info)[j];
-
-    const int ns = md.nshell;
+    const int n = md.nbasis, ns = md.nshell;
     const auto cmc = cptr(md.cutmat);
     const auto cm = gptr(md.cutmat);
-    double dn = 4.0 * cmc[bi.sa * ns + bi.sb];
-    dn = fmax(dn, 4.0 * cm[ki.sa * ns + ki.sb]);
-    dn = fmax(dn, cm[bi.sa * ns + ki.sb]);
-    dn = fmax(dn, cm[bi.sa * ns + ki.sa]);
-    dn = fmax(dn, cm[bi.sb * ns + ki.sa]);
-    dn = fmax(dn, cm[bi.sb * ns + ki.sb]);
-    const double ytest = bi.ycut * ki.ycut;
-    active = active && (ytest > md.cutoff) && (ytest * dn > md.cutoff);
-
-    R acc[C::NACC];
-#pragma unroll
-    for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
-    if (active) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
-
-    R I[C::NI];
-    {
-        const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
-        const R CD[3] = {R(ki.AB[0]), R(ki.AB[1]), R(ki.AB[2])};
-        C::template hrr<R>(acc, AB, CD, I);
-    }
-
-    const int n = md.nbasis;
-    const auto Pc = cptr(md.P);      // bra block: same for every lane
+    const auto Pc = cptr(md.P);
     const auto P = gptr(md.P);
     // spread the atomics of concurrent waves over several copies of G
     const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * n * n);
-    const double s = active ? 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) *
-                                  (bi.id != ki.id ? 2.0 : 1.0)
-                            : 0.0;
-    const double sk = -0.25 * md.hyb * s;
+    const auto bperm = cptr(bb.bperm);
 
-    R Dab[na * nb], Dcd[nc * nd], Dac[na * nc], Dad[na * nd], Dbc[nb * nc], Dbd[nb * nd];
-#pragma unroll
-    for (int a = 0; a < na; ++a)
-#pragma unroll
-        for (int b = 0; b < nb; ++b) Dab[a * nb + b] = R(Pc[(bi.fa + a) * n + bi.fb + b]);
+    // the shared first Q-shell a of every bra of this task
+    const PairInfo b0 = cptr(bb.info)[bperm[t.kb]];
+    const int sa = b0.sa, fa = b0.fa;
+    const double cm_cd = 4.0 * cm[ki.sa * ns + ki.sb];
+    const double cm_ac = cm[sa * ns + ki.sa];
+    const double cm_ad = cm[sa * ns + ki.sb];
+
+    double Dcd[nc * nd], Dac[na * nc], Dad[na * nd];
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = R(P[(ki.fa + c) * n + ki.fb + d]);
+        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = P[(ki.fa + c) * n + ki.fb + d];
 #pragma unroll
     for (int a = 0; a < na; ++a) {
 #pragma unroll
-        for (int c = 0; c < nc; ++c) Dac[a * nc + c] = R(P[(bi.fa + a) * n + ki.fa + c]);
+        for (int c = 0; c < nc; ++c) Dac[a * nc + c] = P[(fa + a) * n + ki.fa + c];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dad[a * nd + d] = R(P[(bi.fa + a) * n + ki.fb + d]);
+        for (int d = 0; d < nd; ++d) Dad[a * nd + d] = P[(fa + a) * n + ki.fb + d];
     }
+    double Jcd[nc * nd], Kac[na * nc], Kad[na * nd];
 #pragma unroll
-    for (int b = 0; b < nb; ++b) {
+    for (int k = 0; k < nc * nd; ++k) Jcd[k] = 0.0;
 #pragma unroll
-        for (int c = 0; c < nc; ++c) Dbc[b * nc + c] = R(P[(bi.fb + b) * n + ki.fa + c]);
+    for (int k = 0; k < na * nc; ++k) Kac[k] = 0.0;
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dbd[b * nd + d] = R(P[(bi.fb + b) * n + ki.fb + d]);
-    }
+    for (int k = 0; k < na * nd; ++k) Kad[k] = 0.0;
+    bool touched = false;
+    const double kscale = -0.25 * md.hyb;
 
-    R Jab[na * nb], Jcd[nc * nd], Kac[na * nc], Kad[na * nd], Kbc[nb * nc], Kbd[nb * nd];
-#pragma unroll
-    for (int k = 0; k < na * nb; ++k) Jab[k] = R(0);
-#pragma unroll
-    for (int k = 0; k < nc * nd; ++k) Jcd[k] = R(0);
-#pragma unroll
-    for (int k = 0; k < na * nc; ++k) Kac[k] = R(0);
-#pragma unroll
-    for (int k = 0; k < na * nd; ++k) Kad[k] = R(0);
-#pragma unroll
-    for (int k = 0; k < nb * nc; ++k) Kbc[k] = R(0);
-#pragma unroll
-    for (int k = 0; k < nb * nd; ++k) Kbd[k] = R(0);
+    for (int kk = t.kb; kk < t.ke; ++kk) {
+        const int i = bperm[kk];
+        const PairInfo bi = cptr(bb.info)[i];
+        const double ytest = bi.ycut * ki.ycut;
+        double dn = fmax(4.0 * cmc[bi.sa * ns + bi.sb], cm_cd);
+        dn = fmax(dn, cm_ad);
+        dn = fmax(dn, cm_ac);
+        dn = fmax(dn, cm[bi.sb * ns + ki.sa]);
+        dn = fmax(dn, cm[bi.sb * ns + ki.sb]);
+        bool valid = in_range && (ytest > md.cutoff) && (ytest * dn > md.cutoff) && (!t.same_block || j <= i);
+        valid = valid && (kFP32 ? (ytest < md.split) : (t.all_fp64 || ytest >= md.split));
+        if (!__any(valid)) continue;
+        touched = touched || valid;
 
+        R acc[C::NACC];
 #pragma unroll
-    for (int a = 0; a < na; ++a)
+        for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
+        if (valid) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
+        R I[C::NI];
+        {
+            const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
+            const R CD[3] = {R(ki.AB[0]), R(ki.AB[1]), R(ki.AB[2])};
+            C::template hrr<R>(acc, AB, CD, I);
+        }
+        const double s = valid ? 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) *
+                                     (bi.id != ki.id ? 2.0 : 1.0)
+                               : 0.0;
+        double Dab[na * nb], Dbc[nb * nc], Dbd[nb * nd];
 #pragma unroll
-        for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < na; ++a)
 #pragma unroll
-            for (int c = 0; c < nc; ++c)
-#pragma unroll
-                for (int d = 0; d < nd; ++d) {
-                    const R cons = R(kCons[C::La][a] * kCons[C::Lb][b] * kCons[C::Lc][c] * kCons[C::Ld][d]);
-                    const R v = I[((a * nb + b) * nc + c) * nd + d] * cons;
-                    Jab[a * nb + b] += Dcd[c * nd + d] * v;
-                    Jcd[c * nd + d] += Dab[a * nb + b] * v;
-                    Kac[a * nc + c] += Dbd[b * nd + d] * v;
-                    Kad[a * nd + d] += Dbc[b * nc + c] * v;
-                    Kbc[b * nc + c] += Dad[a * nd + d] * v;
-                    Kbd[b * nd + d] += Dac[a * nc + c] * v;
-                }
-
-    // bra block: wave-reduced, one atomic per element
-#pragma unroll
-    for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = Pc[(bi.fa + a) * n + bi.fb + b];
 #pragma unroll
         for (int b = 0; b < nb; ++b) {
-            const double v = wave_sum(s * double(Jab[a * nb + b]));
-            if (lane == 0 && v != 0.0) atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+#pragma unroll
+            for (int c = 0; c < nc; ++c) Dbc[b * nc + c] = P[(bi.fb + b) * n + ki.fa + c];
+#pragma unroll
+            for (int d = 0; d < nd; ++d) Dbd[b * nd + d] = P[(bi.fb + b) * n + ki.fb + d];
         }
-    if (!active) return;
+        double Jab[na * nb], Kbc[nb * nc], Kbd[nb * nd];
+#pragma unroll
+        for (int k = 0; k < na * nb; ++k) Jab[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < nb * nc; ++k) Kbc[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < nb * nd; ++k) Kbd[k] = 0.0;
+#pragma unroll
+        for (int a = 0; a < na; ++a)
+#pragma unroll
+            for (int b = 0; b < nb; ++b)
+#pragma unroll
+                for (int c = 0; c < nc; ++c)
+#pragma unroll
+                    for (int d = 0; d < nd; ++d) {
+                        const double cons = kCons[C::La][a] * kCons[C::Lb][b] * kCons[C::Lc][c] * kCons[C::Ld][d];
+                        const double v = double(I[((a * nb + b) * nc + c) * nd + d]) * (cons * s);
+                        Jab[a * nb + b] += Dcd[c * nd + d] * v;
+                        Jcd[c * nd + d] += Dab[a * nb + b] * v;
+                        Kac[a * nc + c] += Dbd[b * nd + d] * v;
+                        Kad[a * nd + d] += Dbc[b * nc + c] * v;
+                        Kbc[b * nc + c] += Dad[a * nd + d] * v;
+                        Kbd[b * nd + d] += Dac[a * nc + c] * v;
+                    }
+        // bra block: summed over the wave, one atomic per element
+#pragma unroll
+        for (int a = 0; a < na; ++a)
+#pragma unroll
+            for (int b = 0; b < nb; ++b) {
+                const double v = wave_sum(Jab[a * nb + b]);
+                if (lane == 0 && v != 0.0) atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+            }
+        if (valid) {
+#pragma unroll
+            for (int b = 0; b < nb; ++b) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fb + b) * n + ki.fa + c], kscale * Kbc[b * nc + c]);
+#pragma unroll
+                for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fb + b) * n + ki.fb + d], kscale * Kbd[b * nd + d]);
+            }
+        }
+    }
+    if (!touched) return;
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], s * double(Jcd[c * nd + d]));
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
 #pragma unroll
     for (int a = 0; a < na; ++a) {
 #pragma unroll
-        for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fa + a) * n + ki.fa + c], sk * double(Kac[a * nc + c]));
+        for (int c = 0; c < nc; ++c) atomic_add(&G[(fa + a) * n + ki.fa + c], kscale * Kac[a * nc + c]);
 #pragma unroll
-        for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fa + a) * n + ki.fb + d], sk * double(Kad[a * nd + d]));
-    }
-#pragma unroll
-    for (int b = 0; b < nb; ++b) {
-#pragma unroll
-        for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fb + b) * n + ki.fa + c], sk * double(Kbc[b * nc + c]));
-#pragma unroll
-        for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fb + b) * n + ki.fb + d], sk * double(Kbd[b * nd + d]));
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(fa + a) * n + ki.fb + d], kscale * Kad[a * nd + d]);
     }
 }
 

@@ -29,6 +29,7 @@ struct PairInfo {
 struct PairBlock {
     const PairInfo* info;   // n entries
     const double* prim;     // PF_N * K * n
+    const int* bperm;       // pair indices grouped by first Q-shell (bra order)
     int n, K, cls, mol;
 };
 
@@ -38,12 +39,17 @@ struct MolDesc {
     const double* cutmat;   // nshell x nshell max |P| per original shell pair
     int nbasis, nshell;
     double cutoff, primlim, hyb;
+    double split;           // quartets with Ycut*Ycut >= split run in FP64, below in FP32
     int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
     int pad;
 };
 
+// One wave: bras bperm[kb..ke) of one bra block - all with the same first
+// Q-shell a, Schwarz-descending - against kets [ket_start, ket_start +
+// ket_count) of one ket block, one ket per lane.
 struct Task {
-    int bra_block, bra_idx, ket_block, ket_start, ket_count, pad0, pad1, pad2;
+    int bra_block, kb, ke, ket_block, ket_start, ket_count, same_block;
+    int all_fp64;   // 1: this class runs every quartet in FP64 even in mixed mode
 };
 
 }  // namespace qf
