@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--cutoff", type=float, default=1e-11)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--cpu-builds", type=int, default=5, help="oracle builds timed for the CPU baseline (median)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus %d is not supported "
@@ -60,6 +63,16 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.copy_probe:
+        nbytes = a.copy_probe << 20
+        x = torch.ones(nbytes, dtype=torch.uint8, device=dev)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "copies": 3}))
+        return
     mol = M.water_cluster(a.nwater)
     bs = B.build_basis(mol, "6-31G*")
     log(f"[bench] (H2O){a.nwater} 6-31G(d): {mol.natom} atoms, {bs.nbasis} functions")
@@ -117,11 +130,23 @@ def main():
     peak = FP32_PEAK_TFLOPS if names[k][1] else FP64_PEAK_TFLOPS
     achieved = flops[k] / (avg_ms[k] * 1e-3) / 1e12
     build_flops = float(np.sum(flops))
+    nquart, nprim = eng.launch_counts()
     per_class = [{"class": nm, "fp32": f32, "tasks": nt, "avg_ms": round(float(m), 4),
-                  "gflop": round(float(fl) / 1e9, 3),
+                  "gflop": round(float(fl) / 1e9, 3), "quartets": int(q), "prim_quartets": int(pq),
                   "tflops": round(float(fl) / (m * 1e-3) / 1e12, 3) if m > 0 else None}
-                 for (nm, f32, nt), m, fl in zip(names, avg_ms, flops)]
+                 for (nm, f32, nt), m, fl, q, pq in zip(names, avg_ms, flops, nquart, nprim)]
     log("[bench] per-class: " + json.dumps(per_class))
+
+    # HBM bytes per launch of that kernel, measured by rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes over this workload (scaled by a copy of known size);
+    # reported only while the summary names the same kernel
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_h2o{a.nwater}_631gd_{a.precision}.json")
+    kname = names[k][0] + (" fp32" if names[k][1] else " fp64")
+    if os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if tj.get("kernel") == kname:
+            traffic = tj["bytes_per_launch"]
 
     # ---- CPU baseline: the oracle, one full build on the host cores ---------
     cpu = None
@@ -130,15 +155,19 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         qb = O.QuickBasisRef(mol.symbols, mol.xyz, "6-31G*")
         orc = O.Oracle(qb)
-        c0 = time.perf_counter()
-        G_cpu, nq = orc.fock2e(P_host, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, threads=threads)
-        tcpu = time.perf_counter() - c0
+        times = []
+        for _ in range(max(1, a.cpu_builds)):
+            c0 = time.perf_counter()
+            G_cpu, nq = orc.fock2e(P_host, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, threads=threads)
+            times.append(time.perf_counter() - c0)
+            log(f"[bench] CPU oracle build {times[-1]:.2f}s on {threads} threads")
+        tcpu = float(np.median(times))
         G_gpu = eng.fock2e(P_host)
         dF = float(np.max(np.abs(G_gpu - G_cpu)))
         cpu = {"value": round(1.0 / tcpu, 5), "unit": "Fock builds/s", "cores": threads, "kind": "port",
-               "sample": f"one full (H2O){a.nwater} 6-31G(d) build, {nq} shell quartets, {tcpu:.2f}s; "
+               "sample": f"median of {len(times)} full (H2O){a.nwater} 6-31G(d) builds of the same density "
+                         f"({nq} shell quartets each, {tcpu:.2f}s per build, OpenMP over bra shells); "
                          f"max|dF| GPU vs CPU {dF:.1e}"}
-        log(f"[bench] CPU oracle build {tcpu:.2f}s on {threads} threads, max|dF| {dF:.2e}")
 
     out = {
         "metric": METRIC,
@@ -156,9 +185,9 @@ def main():
         "config": {"workload": f"(H2O){a.nwater} RHF/6-31G(d) two-electron Fock build G(P), converged density",
                    "natom": mol.natom, "nbasis": n, "integral_cutoff": a.cutoff,
                    "prim_limit": 0.1 * a.cutoff, "precision": a.precision, "seed": M.SEED},
-        "roofline": {"bound": "valu", "kernel": names[k][0] + (" fp32" if names[k][1] else " fp64"),
+        "roofline": {"bound": "valu", "kernel": kname,
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None},
+                     "frac": round(achieved / peak, 4), "traffic": traffic},
         "build_tflops": round(build_flops / (ms_per_step * 1e-3) / 1e12, 3),
         "build_gflop": round(build_flops / 1e9, 3),
         "wall_s_timed": round(wall, 3),

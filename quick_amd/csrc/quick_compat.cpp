@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -35,6 +36,15 @@ QuickState* g_state = nullptr;
 void report(const char* where)
 {
     std::fprintf(stderr, "quickfock: %s failed: %s\n", where, qf_last_error());
+}
+
+// The Fock build has no error argument in QUICK's interface; continuing
+// would hand the SCF a Fock matrix without its two-electron part.
+[[noreturn]] void fatal(const char* where)
+{
+    report(where);
+    std::fflush(stderr);
+    std::exit(EXIT_FAILURE);
 }
 
 }  // namespace
@@ -150,22 +160,22 @@ void gpu_get_cshell_eri_(bool* /*deltaO*/, double* o)
 {
     // deltaO needs no handling: the incremental density already arrives in
     // dense (quick_scf_operator_module.f90:69-79); o += G(dense) as QUICK.
-    if (!g_state || !g_state->eng) return;
+    if (!g_state || !g_state->eng) {
+        std::fprintf(stderr, "quickfock: gpu_get_cshell_eri_ called before gpu_new_/gpu_upload_basis_\n");
+        std::exit(EXIT_FAILURE);
+    }
     if (g_state->oshell) {
         std::fprintf(stderr, "quickfock: open-shell builds are not supported\n");
-        return;
+        std::exit(EXIT_FAILURE);
     }
     if (!g_state->cut_set) {
         qf_set_cutoffs(g_state->eng, g_state->cutoff, g_state->primlim, g_state->hyb);
         g_state->cut_set = true;
     }
     // pair and task lists are rebuilt only after a cutoff or geometry change
-    if (qf_prepare(g_state->eng) != QF_OK) {
-        report("gpu_get_cshell_eri_ (prepare)");
-        return;
-    }
+    if (qf_prepare(g_state->eng) != QF_OK) fatal("gpu_get_cshell_eri_ (prepare)");
     const double* cm = g_state->cutmat.empty() ? nullptr : g_state->cutmat.data();
-    if (qf_fock_build_host(g_state->eng, g_state->dense.data(), o, cm)) report("gpu_get_cshell_eri_");
+    if (qf_fock_build_host(g_state->eng, g_state->dense.data(), o, cm)) fatal("gpu_get_cshell_eri_");
 }
 
 void gpu_cleanup_(void) {}
