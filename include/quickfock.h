@@ -104,11 +104,41 @@ int qf_fock_build_host(qf_engine *e, const double *P, double *F, const double *c
 /* Device pointer of the engine's G (= Gt + Gt^T after the last build). */
 const double *qf_device_g(qf_engine *e);
 
-/* Batched build: one launch per class covers every engine in the list.
- * d_P[i] / d_F[i] are device pointers per engine.  All engines must be
- * prepared and live on the same device. */
+/* One-shot batched build: F_i += G(P_i) for every engine of the list with
+ * one launch per quartet class covering all of them (a temporary qf_batch).
+ * d_P[i] / d_F[i] are device pointers per engine (d_F may be NULL); all
+ * engines live on one device.  Synchronises the stream before returning. */
 int qf_fock_build_batch(qf_engine *const *engines, int count, const double *const *d_P,
                         double *const *d_F, void *stream);
+
+/* Persistent batches (native; no QUICK counterpart; used for many-molecule
+ * SCF iterations and the serving loop).  A batch takes prepared engines
+ * (it prepares them if needed) and owns pooled device buffers: P_i, F_i,
+ * G_i and Cutmatrix_i per molecule.  qf_batch_build adds G(P_i) into F_i
+ * for every molecule with one launch per quartet class; with
+ * qf_batch_set_graph(b, 1) the first build is captured into a HIP graph
+ * and later builds replay it.  The engines must stay alive and unchanged
+ * while the batch exists.  G replicas: 1..64 copies of each G. */
+typedef struct qf_batch qf_batch;
+int qf_batch_create(qf_engine *const *engines, int count, int replicas, qf_batch **out);
+/* Same, with caller-owned device buffers: molecule i reads P from d_P[i] and
+ * adds G into d_F[i] (both nbasis_i^2, alive as long as the batch). */
+int qf_batch_create_ext(qf_engine *const *engines, int count, int replicas, const double *const *d_P,
+                        double *const *d_F, qf_batch **out);
+void qf_batch_destroy(qf_batch *b);
+int qf_batch_size(qf_batch *b);
+double *qf_batch_density(qf_batch *b, int i);   /* device, nbasis_i^2, written by the caller */
+double *qf_batch_fock(qf_batch *b, int i);      /* device, nbasis_i^2, F_i += G_i per build */
+const double *qf_batch_g(qf_batch *b, int i);   /* device, nbasis_i^2, G_i of the last build */
+int qf_batch_set_graph(qf_batch *b, int enable);
+int qf_batch_set_density_host(qf_batch *b, int i, const double *P);   /* synchronous upload */
+int qf_batch_get_host(qf_batch *b, int i, double *G, double *F);       /* synchronous; either may be NULL */
+int qf_batch_clear_fock(qf_batch *b, void *stream);                     /* F_i = 0 for every molecule */
+int qf_batch_build(qf_batch *b, void *stream);
+int qf_batch_launch_count(qf_batch *b);
+int qf_batch_launch_info(qf_batch *b, int idx, int *cb, int *ck, int *fp32, int *ntasks);
+int qf_batch_set_timing(qf_batch *b, int enable);   /* per-launch events; builds run eagerly */
+int qf_batch_read_timing(qf_batch *b, double *ms_per_launch, int *calls_per_launch);
 
 /* One-electron matrices for the SCF harness: S, T, V (nbasis*nbasis, host)
  * with nuclear charges[natom]; computed on the GPU. */
@@ -130,6 +160,25 @@ int qf_launch_counts(qf_engine *e, long long *quartets, long long *prim_quartets
  * `replicas` copies of G (summed when the build finishes), and tasks inside
  * a launch are permuted so that concurrent waves touch different kets. */
 int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
+
+/* Exchange digestion through LDS tiles (native; no QUICK counterpart):
+ * enable = 1 accumulates J_ab and all exchange terms of a bra chunk in a
+ * workgroup-shared tile and adds it to G once, instead of one FP64 atomic
+ * per quartet.  waves: waves per workgroup (1..4); tasks_per_group: tasks
+ * one workgroup takes; lds_bytes: tile budget (1 KiB..64 KiB).  Classes
+ * whose tile does not fit fall back to direct atomics. */
+int qf_set_tile(qf_engine *e, int enable, int waves, int tasks_per_group, int lds_bytes);
+
+/* Per-class launch shape (native; used by the autotuner): for the quartet
+ * class (cb|ck), pair classes 0..5 = ss, ps, pp, ds, dp, dd with ck <= cb:
+ * tile (-1 engine default, 0 direct atomics, 1 LDS tile), waves per tile
+ * workgroup, tasks per tile workgroup and bras per task (0: default). */
+int qf_set_class_tuning(qf_engine *e, int cb, int ck, int tile, int waves, int tasks_per_group,
+                        int bras_per_task);
+
+/* Persistent launches: at most max_workgroups workgroups per class launch,
+ * each striding over the tasks (0: one workgroup per task / tile group). */
+int qf_set_persistent(qf_engine *e, int max_workgroups);
 
 /* HIP-event timing of every class launch (on the build stream); read sums
  * the durations per launch index since the last read. */

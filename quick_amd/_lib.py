@@ -34,6 +34,15 @@ def lib():
         return _lib
     if not os.path.exists(LIBPATH):
         raise QuickFockError(f"{LIBPATH} not found: build it with `python -m quick_amd.build`")
+    # PyTorch ships its own HIP runtime under the same soname.  Loading it
+    # first makes this library bind to that copy, so the process holds one
+    # HIP runtime and torch streams / device pointers are valid here.
+    # (Loaded the other way round, torch would bring a second runtime that
+    # cannot open the device.)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIBPATH)
     vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     sig = {
@@ -61,7 +70,27 @@ def lib():
         "qf_launch_flops": (ctypes.c_int, [vp, dp]),
         "qf_launch_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
         "qf_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+        "qf_batch_create_ext": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp),
+                                                ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "qf_batch_destroy": (None, [vp]),
+        "qf_batch_size": (ctypes.c_int, [vp]),
+        "qf_batch_density": (vp, [vp, ctypes.c_int]),
+        "qf_batch_fock": (vp, [vp, ctypes.c_int]),
+        "qf_batch_g": (vp, [vp, ctypes.c_int]),
+        "qf_batch_set_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_batch_set_density_host": (ctypes.c_int, [vp, ctypes.c_int, dp]),
+        "qf_batch_get_host": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
+        "qf_batch_clear_fock": (ctypes.c_int, [vp, vp]),
+        "qf_batch_build": (ctypes.c_int, [vp, vp]),
+        "qf_batch_launch_count": (ctypes.c_int, [vp]),
+        "qf_batch_launch_info": (ctypes.c_int, [vp, ctypes.c_int, ip, ip, ip, ip]),
+        "qf_batch_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_batch_read_timing": (ctypes.c_int, [vp, dp, ip]),
         "qf_set_tuning": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "qf_set_tile": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "qf_set_class_tuning": (ctypes.c_int, [vp] + [ctypes.c_int] * 6),
+        "qf_set_persistent": (ctypes.c_int, [vp, ctypes.c_int]),
         "qf_read_timing": (ctypes.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():

@@ -1,0 +1,81 @@
+"""Per-class launch-shape autotuner for the Fock build.
+
+Every class launch of a build is timed with HIP events (launches run one
+after another on one stream, so a class's time does not depend on the shape
+chosen for the others).  Each candidate shape is applied to all classes at
+once, a few builds are timed, and each (bra|ket) class then keeps the
+candidate that was fastest for it.  Works on a FockEngine (one molecule) or
+on a list of engines that are built as a FockBatch.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# (tile, waves, tasks_per_group, bras_per_task); 0 = engine default
+CANDIDATES = [
+    (0, 0, 0, 0),
+    (1, 4, 16, 0),
+    (1, 4, 8, 0),
+    (1, 2, 8, 0),
+    (1, 1, 4, 0),
+    (0, 0, 0, 4),
+    (1, 4, 32, 0),
+]
+
+
+def _class_times(build, launches, read_timing, set_timing, builds):
+    set_timing(True)
+    for _ in range(builds):
+        build()
+    ms, calls = read_timing()
+    set_timing(False)
+    out = {}
+    for (name, f32, _), m, c in zip(launches(), ms, calls):
+        out[(name, f32)] = out.get((name, f32), 0.0) + (m / max(c, 1))
+    return out
+
+
+def _class_index(name):
+    names = ["ss", "ps", "pp", "ds", "dp", "dd"]
+    a, b = name.strip("()").split("|")
+    return names.index(a), names.index(b)
+
+
+def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None):
+    """Choose a launch shape per class.  ``engines``: one FockEngine or a
+    list (tuned as one FockBatch).  ``make_density(i)`` returns a device
+    pointer (int) of a density for engine i (single engine) or is ignored
+    for batches (their density slots are used as they are).  Returns
+    {class name: (candidate, ms)} and leaves the best shapes applied."""
+    from .engine import FockBatch
+    single = not isinstance(engines, (list, tuple))
+    engs = [engines] if single else list(engines)
+    times = {}
+    for cand in candidates:
+        for e in engs:
+            for cb in range(6):
+                for ck in range(cb + 1):
+                    e.set_class_tuning(cb, ck, *cand)
+        if single:
+            e = engs[0]
+            P = make_density(0)
+            times[cand] = _class_times(lambda: e.fock2e_device(P, 0, stream), e.launches, e.read_timing,
+                                       e.set_timing, builds)
+        else:
+            b = FockBatch(engs, **(batch_kw or {}))
+            times[cand] = _class_times(lambda: b.build(stream), b.launches, b.read_timing, b.set_timing, builds)
+            b.close()
+    # per class the candidate with the smallest time summed over its FP64 and
+    # FP32 launches (they share one shape)
+    names = sorted({name for t in times.values() for (name, _) in t})
+    chosen = {}
+    for name in names:
+        def total(c):
+            keys = [k for k in times[c] if k[0] == name]
+            return sum(times[c][k] for k in keys) if keys else float("inf")
+        cand = min(candidates, key=total)
+        cb, ck = _class_index(name)
+        for e in engs:
+            e.set_class_tuning(cb, ck, *cand)
+        chosen[name] = (cand, total(cand))
+    return chosen

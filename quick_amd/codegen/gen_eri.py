@@ -326,12 +326,24 @@ def launcher_source(name, diag):
     out = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.",
            '#include "%s.h"' % name, "", "namespace qf {", "",
            "void launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
-           " hipStream_t s)" % name, "{",
+           " int grid, hipStream_t s)" % name, "{",
            "    if (nt <= 0) return;",
+           "    if (grid <= 0 || grid > nt) grid = nt;",
            "    if (fp32)",
-           "        hipLaunchKernelGGL((fock_kernel<%s, float>), dim3(nt), dim3(WAVE), 0, s, t, b, m);" % name,
+           "        hipLaunchKernelGGL((fock_kernel<%s, float>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
            "    else",
-           "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(nt), dim3(WAVE), 0, s, t, b, m);" % name,
+           "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
+           "}", "",
+           "void launch_fock_tile_%s(const TileGroup* g, int ng, const Task* t, const PairBlock* b,"
+           " const MolDesc* m, int fp32, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
+           "    if (ng <= 0) return;",
+           "    if (grid <= 0 || grid > ng) grid = ng;",
+           "    if (fp32)",
+           "        hipLaunchKernelGGL((fock_tile_kernel<%s, float>), dim3(grid), dim3(waves * WAVE), lds, s,"
+           " g, ng, t, b, m);" % name,
+           "    else",
+           "        hipLaunchKernelGGL((fock_tile_kernel<%s, double>), dim3(grid), dim3(waves * WAVE), lds, s,"
+           " g, ng, t, b, m);" % name,
            "}", ""]
     if diag:
         out += ["void launch_schwarz_%s(PairBlock blk, double* out, hipStream_t s)" % name, "{",
@@ -373,11 +385,15 @@ def main():
     pc = {pcl: k for k, pcl in enumerate(PAIR_CLASSES)}
     lines = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.", "#pragma once",
              "#include <hip/hip_runtime.h>", '#include "../qf_types.h"', "", "namespace qf {", "",
-             "using FockLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);",
+             "using FockLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
+             "using FockTileLaunch = void (*)(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
+             " int, int, size_t, int, hipStream_t);",
              "using SchwarzLaunch = void (*)(PairBlock, double*, hipStream_t);", ""]
     for (name, La, Lb, Lc, Ld, diag) in names:
-        lines.append("void launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);"
+        lines.append("void launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
                      % name)
+        lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
+                     " int, int, size_t, int, hipStream_t);" % name)
         if diag:
             lines.append("void launch_schwarz_%s(PairBlock, double*, hipStream_t);" % name)
     # FLOP model (FMA = 2): per surviving primitive quartet the VRR plus
@@ -397,6 +413,9 @@ def main():
               "inline FockLaunch fock_launcher(int cb, int ck)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
         lines.append("    if (cb == %d && ck == %d) return launch_fock_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
+    lines += ["    return nullptr;", "}", "", "inline FockTileLaunch fock_tile_launcher(int cb, int ck)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        lines.append("    if (cb == %d && ck == %d) return launch_fock_tile_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
     lines += ["    return nullptr;", "}", "", "inline SchwarzLaunch schwarz_launcher(int c)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
         if diag:

@@ -87,6 +87,7 @@ struct HostBlock {
     std::vector<std::pair<int, int>> groups;     // [start, end) in bperm, one per first Q-shell
     std::vector<double> prim;   // PF_N * K * n, filled at finalisation
     std::vector<std::vector<double>> rows;   // per pair: PF_N*K values (field-major)
+    std::vector<int> colf;                   // compact column -> basis function
 };
 
 // --------------------------------------------------------------------------
@@ -190,12 +191,24 @@ struct qf_engine {
     double split_thr = 0.0;  // Ycut_ij*Ycut_kl below which quartets run in FP32
     int replicas = 8;        // copies of G the atomics are spread over
     int shuffle = 1;         // permute tasks inside a launch (decorrelates atomic targets)
+    int tile = 1;            // accumulate exchange terms in LDS tiles (fock_tile_kernel)
+    int tile_waves = 4;      // max waves per tile workgroup
+    int tile_tasks = 16;     // max tasks per tile workgroup
+    int lds_budget = 64 * 1024;
+    int persist = 0;         // grid cap of the class launches (0: one workgroup per task / group)
+    struct ClassTune {
+        int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
+    };
+    ClassTune ctune[6][6];
     // pair blocks
     std::vector<HostBlock> blocks;
     std::vector<qf::PairBlock> hblocks;
     std::vector<qf::Task> tasks;
+    std::vector<qf::TileGroup> groups;
     struct Launch {
         int cb, ck, fp32, offset, count;
+        int tile, goffset, gcount, waves;
+        size_t lds;
     };
     std::vector<Launch> launches;
     bool prepared = false;
@@ -208,6 +221,8 @@ struct qf_engine {
     DevBuf<qf::PairBlock> d_blocks;
     DevBuf<qf::MolDesc> d_mol;
     DevBuf<qf::Task> d_tasks;
+    DevBuf<qf::TileGroup> d_groups;
+    DevBuf<int> d_colf;
     DevBuf<double> d_Gt, d_Gs, d_cm, d_P;
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
@@ -272,7 +287,7 @@ void make_pair(const qf_engine* e, int qa, int qb, int ip, int jp, bool use_cut,
         info.AB[d] = A[d] - B[d];
         AB2 += info.AB[d] * info.AB[d];
     }
-    info.pad = 0.0;
+    info.cola = info.colb = 0;
     struct Prim {
         double v[qf::PF_N];
     };
@@ -305,7 +320,8 @@ void make_pair(const qf_engine* e, int qa, int qb, int ip, int jp, bool use_cut,
 
 // Assemble host blocks into flat device arrays; fills hblocks (device pointers).
 int upload_blocks(qf_engine* e, std::vector<HostBlock>& blocks, DevBuf<qf::PairInfo>& d_info,
-                  DevBuf<double>& d_prim, std::vector<qf::PairBlock>& out, int mol, DevBuf<int>* d_perm = nullptr)
+                  DevBuf<double>& d_prim, std::vector<qf::PairBlock>& out, int mol, DevBuf<int>* d_perm = nullptr,
+                  DevBuf<int>* d_colf = nullptr)
 {
     size_t ninfo = 0, nprimv = 0;
     for (auto& b : blocks) {
@@ -336,12 +352,22 @@ int upload_blocks(qf_engine* e, std::vector<HostBlock>& blocks, DevBuf<qf::PairI
         perm.insert(perm.end(), b.bperm.begin(), b.bperm.end());
     }
     if (d_perm) QF_HIP(d_perm->upload(perm));
+    std::vector<int> colf;
+    std::vector<size_t> coloff;
+    for (auto& b : blocks) {
+        coloff.push_back(colf.size());
+        colf.insert(colf.end(), b.colf.begin(), b.colf.end());
+    }
+    if (d_colf) QF_HIP(d_colf->upload(colf));
     out.clear();
     for (size_t k = 0; k < blocks.size(); ++k) {
         qf::PairBlock pb;
         pb.info = d_info.p + ioff[k];
         pb.prim = d_prim.p + poff[k];
         pb.bperm = (d_perm && !blocks[k].bperm.empty()) ? d_perm->p + permoff[k] : nullptr;
+        pb.colf = (d_colf && !blocks[k].colf.empty()) ? d_colf->p + coloff[k] : nullptr;
+        pb.ncol = (int)blocks[k].colf.size();
+        pb.pad = 0;
         pb.n = (int)blocks[k].info.size();
         pb.K = blocks[k].K;
         pb.cls = blocks[k].cls;
@@ -525,7 +551,27 @@ int prepare(qf_engine* e)
             k = e2;
         }
     }
-    rc = upload_blocks(e, e->blocks, e->d_info, e->d_prim, e->hblocks, 0, &e->d_perm);
+    for (auto& b : e->blocks) {
+        // compact columns: the functions of every Q-shell of the block, by Q-shell id
+        std::vector<int> qlist;
+        for (const auto& p : b.info) {
+            qlist.push_back(p.qa);
+            qlist.push_back(p.qb);
+        }
+        std::sort(qlist.begin(), qlist.end());
+        qlist.erase(std::unique(qlist.begin(), qlist.end()), qlist.end());
+        std::vector<int> qcol(e->qs.size(), -1);
+        b.colf.clear();
+        for (int q : qlist) {
+            qcol[q] = (int)b.colf.size();
+            for (int f = 0; f < ncart(e->qs[q].L); ++f) b.colf.push_back(e->qs[q].fn + f);
+        }
+        for (auto& p : b.info) {
+            p.cola = qcol[p.qa];
+            p.colb = qcol[p.qb];
+        }
+    }
+    rc = upload_blocks(e, e->blocks, e->d_info, e->d_prim, e->hblocks, 0, &e->d_perm, &e->d_colf);
     if (rc) return rc;
     for (auto& b : e->blocks) b.rows.clear();
     QF_HIP(e->d_blocks.upload(e->hblocks));
@@ -534,6 +580,7 @@ int prepare(qf_engine* e)
     const double thr_hi = e->split * ymax * ymax;
     e->split_thr = thr_hi;
     e->tasks.clear();
+    e->groups.clear();
     e->launches.clear();
     e->stats = qf_stats{};
     for (auto& b : e->blocks) e->stats.n_pairs += (long long)b.info.size();
@@ -547,6 +594,27 @@ int prepare(qf_engine* e)
                                          kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
                 if (fp32 && !split_class) continue;
                 const int offset = (int)e->tasks.size();
+                const int goffset = (int)e->groups.size();
+                const int na = ncart(kClassL[cb][0]), nb = ncart(kClassL[cb][1]);
+                // bras per task: long loops for cheap classes (more register
+                // accumulation, fewer atomics), short ones for expensive classes
+                // (enough waves to fill 256 CUs)
+                const int ni = na * nb * ncart(kClassL[ck][0]) * ncart(kClassL[ck][1]);
+                const qf_engine::ClassTune& ct = e->ctune[cb][ck];
+                int mmax = ni <= 10 ? 16 : ni <= 36 ? 8 : ni <= 108 ? 4 : ni <= 324 ? 2 : 1;
+                if (ct.mmax > 0) mmax = ct.mmax;
+                const int tile_tasks = ct.tpg > 0 ? ct.tpg : e->tile_tasks;
+                const int tile_waves = ct.waves > 0 ? ct.waves : e->tile_waves;
+                // LDS tile: (na + m*nb) x ncol exchange terms + m*na*nb Coulomb terms
+                int maxcol = 0;
+                for (const auto& Kb : e->blocks)
+                    if (Kb.cls == ck) maxcol = std::max(maxcol, (int)Kb.colf.size());
+                const long long room = e->lds_budget / 8 - (long long)na * maxcol;
+                const long long mfit = room > 0 ? room / ((long long)nb * maxcol + na * nb) : 0;
+                const bool tile = (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0) && mfit >= 1;
+                if (tile) mmax = (int)std::min<long long>(mmax, mfit);
+                size_t lds = 0;
+                long long group_tasks = 0;
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {
                     const HostBlock& B = e->blocks[bbk];
                     if (B.cls != cb) continue;
@@ -555,12 +623,6 @@ int prepare(qf_engine* e)
                         if (Kb.cls != ck) continue;
                         if (cb == ck && kbk > bbk) continue;
                         const bool same = (cb == ck && kbk == bbk);
-                        // bras per task: long loops for cheap classes (more register
-                        // accumulation, fewer atomics), short ones for expensive classes
-                        // (enough waves to fill 256 CUs)
-                        const int ni = ncart(kClassL[cb][0]) * ncart(kClassL[cb][1]) * ncart(kClassL[ck][0]) *
-                                       ncart(kClassL[ck][1]);
-                        const int mmax = ni <= 10 ? 16 : ni <= 36 ? 8 : ni <= 108 ? 4 : ni <= 324 ? 2 : 1;
                         std::vector<std::pair<int, int>> chunks;
                         for (const auto& g : B.groups)
                             for (int c0 = g.first; c0 < g.second; c0 += mmax)
@@ -582,6 +644,7 @@ int prepare(qf_engine* e)
                                 hi[k] = fp32 ? n : n64;
                                 nmax = std::max(nmax, hi[k]);
                             }
+                            const int chunk_first = (int)e->tasks.size();
                             for (int s0 = 0; s0 < nmax; s0 += qf::WAVE) {
                                 int kb = -1, ke = -1;
                                 for (int k = 0; k < M; ++k)
@@ -601,11 +664,41 @@ int prepare(qf_engine* e)
                                 t.all_fp64 = split_class ? 0 : 1;
                                 e->tasks.push_back(t);
                             }
+                            if (!tile) continue;
+                            // workgroups of the chunk: consecutive runs of its tasks
+                            const int chunk_end = (int)e->tasks.size();
+                            for (int t0 = chunk_first; t0 < chunk_end; t0 += tile_tasks) {
+                                qf::TileGroup tg{};
+                                tg.task_begin = t0;
+                                tg.task_end = std::min(chunk_end, t0 + tile_tasks);
+                                tg.chunk_kb = g.first;
+                                tg.nbra = M;
+                                e->groups.push_back(tg);
+                                group_tasks += tg.task_end - tg.task_begin;
+                            }
+                            if (chunk_end > chunk_first)
+                                lds = std::max(lds, sizeof(double) * ((size_t)(na + M * nb) * Kb.colf.size() +
+                                                                      (size_t)M * na * nb));
                         }
                     }
                 }
                 const int count = (int)e->tasks.size() - offset;
-                if (e->shuffle && count > 1) {
+                const int gcount = (int)e->groups.size() - goffset;
+                auto cost = [](const qf::Task& x) { return (long long)(x.ke - x.kb) * x.ket_count; };
+                if (tile && gcount > 0) {
+                    // longest workgroups first so the launch ends evenly
+                    std::vector<long long> gc(gcount, 0);
+                    for (int q = 0; q < gcount; ++q) {
+                        const auto& tg = e->groups[goffset + q];
+                        for (int x = tg.task_begin; x < tg.task_end; ++x) gc[q] += cost(e->tasks[x]);
+                    }
+                    std::vector<int> ord(gcount);
+                    std::iota(ord.begin(), ord.end(), 0);
+                    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return gc[x] > gc[y]; });
+                    std::vector<qf::TileGroup> sorted(gcount);
+                    for (int q = 0; q < gcount; ++q) sorted[q] = e->groups[goffset + ord[q]];
+                    std::copy(sorted.begin(), sorted.end(), e->groups.begin() + goffset);
+                } else if (e->shuffle && count > 1) {
                     // deterministic permutation (concurrent waves then hit different
                     // kets), then longest tasks first so the launch ends evenly
                     uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)(cb * 12 + ck * 2 + fp32);
@@ -616,14 +709,49 @@ int prepare(qf_engine* e)
                         const int r = (int)(st % (uint64_t)(q + 1));
                         std::swap(e->tasks[offset + q], e->tasks[offset + r]);
                     }
-                    std::stable_sort(e->tasks.begin() + offset, e->tasks.end(), [](const qf::Task& x, const qf::Task& y) {
-                        return (x.ke - x.kb) * x.ket_count > (y.ke - y.kb) * y.ket_count;
-                    });
+                    std::stable_sort(e->tasks.begin() + offset, e->tasks.end(),
+                                     [&](const qf::Task& x, const qf::Task& y) { return cost(x) > cost(y); });
                 }
-                if (count > 0) e->launches.push_back({cb, ck, fp32, offset, count});
+                if (count > 0) {
+                    qf_engine::Launch L{};
+                    L.cb = cb;
+                    L.ck = ck;
+                    L.fp32 = fp32;
+                    L.offset = offset;
+                    L.count = count;
+                    L.tile = tile && gcount > 0;
+                    L.goffset = goffset;
+                    L.gcount = gcount;
+                    L.waves = gcount > 0 ? (int)std::min<long long>(tile_waves,
+                                                                  std::max<long long>(1, (group_tasks + gcount - 1) / gcount))
+                                         : 1;
+                    L.lds = lds;
+                    e->launches.push_back(L);
+                }
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
             }
+    // every tile workgroup must stay inside its chunk and its LDS allocation
+    for (const auto& L : e->launches) {
+        if (!L.tile) continue;
+        const int na = ncart(kClassL[L.cb][0]), nb = ncart(kClassL[L.cb][1]);
+        for (int q = L.goffset; q < L.goffset + L.gcount; ++q) {
+            const qf::TileGroup& g = e->groups[q];
+            if (g.task_begin < L.offset || g.task_end > L.offset + L.count || g.task_begin >= g.task_end)
+                return fail(QF_ERR_STATE, "tile group outside its launch");
+            const qf::Task& t0 = e->tasks[g.task_begin];
+            const size_t need = sizeof(double) * ((size_t)(na + g.nbra * nb) * e->blocks[t0.ket_block].colf.size() +
+                                                  (size_t)g.nbra * na * nb);
+            if (need > L.lds) return fail(QF_ERR_STATE, "tile group exceeds its LDS allocation");
+            for (int x = g.task_begin; x < g.task_end; ++x) {
+                const qf::Task& t = e->tasks[x];
+                if (t.bra_block != t0.bra_block || t.ket_block != t0.ket_block || t.kb < g.chunk_kb ||
+                    t.ke > g.chunk_kb + g.nbra || t.kb >= t.ke)
+                    return fail(QF_ERR_STATE, "tile group mixes chunks");
+            }
+        }
+    }
     QF_HIP(e->d_tasks.upload(e->tasks));
+    QF_HIP(e->d_groups.upload(e->groups));
 
     // ---- per-build buffers -------------------------------------------------
     const size_t n2 = (size_t)e->nbasis * e->nbasis;
@@ -689,7 +817,14 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
             e->ev_launch.push_back((int)li);
             QF_HIP(hipEventRecord(ea, s));
         }
-        f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, L.fp32, s);
+        if (L.tile) {
+            qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
+            if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
+            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, e->d_mol.p, L.fp32, L.waves, L.lds,
+               e->persist, s);
+        } else {
+            f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, L.fp32, e->persist, s);
+        }
         if (e->timing) QF_HIP(hipEventRecord(eb, s));
     }
     hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, e->d_Gt.p, e->replicas, e->d_Gs.p,
@@ -880,16 +1015,6 @@ int qf_internal_basis(qf_engine* e, int* natom, const double** xyz, int* nshell,
     return QF_OK;
 }
 
-int qf_fock_build_batch(qf_engine* const* engines, int count, const double* const* d_P, double* const* d_F,
-                        void* stream)
-{
-    for (int k = 0; k < count; ++k) {
-        int rc = qf_fock_build_device(engines[k], d_P[k], d_F ? d_F[k] : nullptr, nullptr, stream);
-        if (rc) return rc;
-    }
-    return QF_OK;
-}
-
 double qf_flops_estimate(qf_engine* e)
 {
     const int n = qf_launch_count(e);
@@ -966,6 +1091,41 @@ int qf_set_tuning(qf_engine* e, int replicas, int shuffle)
     return QF_OK;
 }
 
+int qf_set_tile(qf_engine* e, int enable, int waves, int tasks_per_group, int lds_bytes)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    if (waves < 1 || waves > 4) return fail(QF_ERR_ARG, "waves must be in 1..4");
+    if (tasks_per_group < 1) return fail(QF_ERR_ARG, "tasks_per_group must be positive");
+    if (lds_bytes < 1024 || lds_bytes > 64 * 1024) return fail(QF_ERR_ARG, "lds_bytes must be in 1 KiB..64 KiB");
+    e->tile = enable != 0;
+    e->tile_waves = waves;
+    e->tile_tasks = tasks_per_group;
+    e->lds_budget = lds_bytes;
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int tasks_per_group, int bras_per_task)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (waves > 4 || bras_per_task > 64) return fail(QF_ERR_ARG, "waves <= 4 and bras_per_task <= 64");
+    qf_engine::ClassTune& t = e->ctune[cb][ck];
+    t.tile = tile < 0 ? -1 : (tile != 0);
+    t.waves = std::max(0, waves);
+    t.tpg = std::max(0, tasks_per_group);
+    t.mmax = std::max(0, bras_per_task);
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_persistent(qf_engine* e, int max_workgroups)
+{
+    if (!e || max_workgroups < 0) return fail(QF_ERR_ARG, "max_workgroups must be >= 0");
+    e->persist = max_workgroups;
+    return QF_OK;
+}
+
 int qf_set_timing(qf_engine* e, int enable)
 {
     if (!e) return fail(QF_ERR_ARG, "NULL");
@@ -994,6 +1154,450 @@ int qf_read_timing(qf_engine* e, double* ms_per_launch, int* calls_per_launch)
     }
     e->ev_used = 0;
     e->ev_launch.clear();
+    return QF_OK;
+}
+
+}  // extern "C"
+
+// ==========================================================================
+// Batches: many prepared engines (molecules) built together, one launch per
+// quartet class for all of them, optionally replayed from a HIP graph.
+// Each molecule keeps its own pair data (owned by its engine); the batch
+// owns the combined block/task/group tables and pooled P, F, G and Cutmatrix
+// buffers, so every pointer a launch uses is fixed and the whole build can be
+// captured once and replayed.
+// ==========================================================================
+namespace {
+
+struct BatchMol {
+    const double* P;
+    double* F;
+    double* Gt;      // replicas * n * n
+    double* Gs;      // n * n
+    double* cm;      // ns * ns
+    const int* f0;
+    const int* f1;
+    int n, ns;
+};
+
+__global__ void k_cutmat_batch(const BatchMol* __restrict__ bm)
+{
+    const BatchMol m = bm[blockIdx.y];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m.ns * m.ns) return;
+    const int I = idx / m.ns, J = idx % m.ns;
+    double v = 0.0;
+    for (int i = m.f0[I]; i <= m.f1[I]; ++i)
+        for (int j = m.f0[J]; j <= m.f1[J]; ++j) v = fmax(v, fabs(m.P[i * m.n + j]));
+    m.cm[idx] = v;
+}
+
+__global__ void k_add(double* __restrict__ y, const double* __restrict__ x, int count)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) y[idx] += x[idx];
+}
+
+__global__ void k_finalize_batch(const BatchMol* __restrict__ bm, int replicas)
+{
+    const BatchMol m = bm[blockIdx.y];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m.n * m.n) return;
+    const int i = idx / m.n, j = idx % m.n;
+    const size_t n2 = (size_t)m.n * m.n;
+    double g = 0.0;
+    for (int r = 0; r < replicas; ++r) g += m.Gt[r * n2 + i * m.n + j] + m.Gt[r * n2 + j * m.n + i];
+    m.Gs[idx] = g;
+    m.F[idx] += g;
+}
+
+}  // namespace
+
+struct qf_batch {
+    int device = 0;
+    std::vector<qf_engine*> engs;
+    int replicas = 2;
+    int persist = 0;
+    std::vector<qf_engine::Launch> launches;
+    std::vector<BatchMol> mols;
+    DevBuf<BatchMol> d_bm;
+    DevBuf<qf::PairBlock> d_blocks;
+    DevBuf<qf::Task> d_tasks;
+    DevBuf<qf::TileGroup> d_groups;
+    DevBuf<qf::MolDesc> d_moldesc;
+    DevBuf<double> d_P, d_F, d_Gt, d_Gs, d_cm;
+    size_t gt_total = 0;
+    int max_n2 = 0, max_ns2 = 0;
+    // graph replay
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    // per-launch timing (eager builds only)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> ev_launch;
+    ~qf_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+    }
+};
+
+namespace {
+
+int batch_issue(qf_batch* b, hipStream_t s)
+{
+    const int nm = (int)b->mols.size();
+    hipLaunchKernelGGL(k_cutmat_batch, dim3((b->max_ns2 + 255) / 256, nm), dim3(256), 0, s, b->d_bm.p);
+    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, b->d_Gt.p, b->gt_total);
+    for (size_t li = 0; li < b->launches.size(); ++li) {
+        const auto& L = b->launches[li];
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (b->timing) {
+            while (b->ev_pool.size() < b->ev_used + 2) {
+                hipEvent_t ev;
+                QF_HIP(hipEventCreate(&ev));
+                b->ev_pool.push_back(ev);
+            }
+            ea = b->ev_pool[b->ev_used];
+            eb = b->ev_pool[b->ev_used + 1];
+            b->ev_used += 2;
+            b->ev_launch.push_back((int)li);
+            QF_HIP(hipEventRecord(ea, s));
+        }
+        if (L.tile) {
+            qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
+            if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
+            ft(b->d_groups.p + L.goffset, L.gcount, b->d_tasks.p, b->d_blocks.p, b->d_moldesc.p, L.fp32, L.waves,
+               L.lds, b->persist, s);
+        } else {
+            qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
+            if (!f) return fail(QF_ERR_ARG, "no kernel for class");
+            f(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, L.fp32, b->persist, s);
+        }
+        if (b->timing) QF_HIP(hipEventRecord(eb, s));
+    }
+    hipLaunchKernelGGL(k_finalize_batch, dim3((b->max_n2 + 255) / 256, nm), dim3(256), 0, s, b->d_bm.p,
+                       b->replicas);
+    QF_HIP(hipGetLastError());
+    return QF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, const double* const* d_P,
+                        double* const* d_F, qf_batch** out)
+{
+    if (!engines || count <= 0 || !out) return fail(QF_ERR_ARG, "engines/count/out");
+    if ((d_P == nullptr) != (d_F == nullptr)) return fail(QF_ERR_ARG, "give both d_P and d_F or neither");
+    if (replicas < 1 || replicas > 64) return fail(QF_ERR_ARG, "replicas must be in 1..64");
+    *out = nullptr;
+    auto* b = new qf_batch();
+    b->device = engines[0]->device;
+    b->replicas = replicas;
+    b->persist = engines[0]->persist;
+    int rc = QF_OK;
+    auto bail = [&](int code) {
+        delete b;
+        return code;
+    };
+    if (hipSetDevice(b->device) != hipSuccess) return bail(fail(QF_ERR_BADDEVICE, "hipSetDevice"));
+    for (int i = 0; i < count; ++i) {
+        qf_engine* e = engines[i];
+        if (!e) return bail(fail(QF_ERR_ARG, "NULL engine"));
+        if (e->device != b->device) return bail(fail(QF_ERR_ARG, "engines of a batch must share one device"));
+        rc = prepare(e);
+        if (rc) return bail(rc);
+        b->engs.push_back(e);
+    }
+    // pooled buffers
+    size_t np = 0, ncm = 0;
+    std::vector<size_t> poff, cmoff, gtoff;
+    for (qf_engine* e : b->engs) {
+        const size_t n2 = (size_t)e->nbasis * e->nbasis;
+        poff.push_back(np);
+        cmoff.push_back(ncm);
+        gtoff.push_back(b->gt_total);
+        np += n2;
+        ncm += (size_t)e->nshell * e->nshell;
+        b->gt_total += n2 * replicas;
+        b->max_n2 = std::max<int>(b->max_n2, (int)n2);
+        b->max_ns2 = std::max<int>(b->max_ns2, e->nshell * e->nshell);
+    }
+    if ((!d_P && (b->d_P.resize(np) != hipSuccess || b->d_F.resize(np) != hipSuccess)) || b->d_Gs.resize(np) != hipSuccess || b->d_cm.resize(ncm) != hipSuccess ||
+        b->d_Gt.resize(b->gt_total) != hipSuccess)
+        return bail(fail(QF_ERR_MEMORY, "batch buffers"));
+    if (!d_P && (hipMemset(b->d_P.p, 0, np * sizeof(double)) != hipSuccess ||
+                 hipMemset(b->d_F.p, 0, np * sizeof(double)) != hipSuccess))
+        return bail(fail(QF_ERR_HIP, "hipMemset"));
+    // combined tables
+    std::vector<qf::PairBlock> blocks;
+    std::vector<qf::MolDesc> md;
+    std::vector<int> boff;
+    for (int i = 0; i < count; ++i) {
+        qf_engine* e = b->engs[i];
+        boff.push_back((int)blocks.size());
+        for (qf::PairBlock pb : e->hblocks) {
+            pb.mol = i;
+            blocks.push_back(pb);
+        }
+        if (d_P && (!d_P[i] || !d_F[i])) return bail(fail(QF_ERR_ARG, "NULL density/Fock pointer"));
+        qf::MolDesc m{};
+        m.P = d_P ? d_P[i] : b->d_P.p + poff[i];
+        m.G = b->d_Gt.p + gtoff[i];
+        m.cutmat = b->d_cm.p + cmoff[i];
+        m.nbasis = e->nbasis;
+        m.nshell = e->nshell;
+        m.cutoff = e->cutoff;
+        m.primlim = e->primlim;
+        m.hyb = e->hyb;
+        m.replicas = replicas;
+        m.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
+        md.push_back(m);
+        BatchMol bm{};
+        bm.P = m.P;
+        bm.F = d_F ? d_F[i] : b->d_F.p + poff[i];
+        bm.Gt = m.G;
+        bm.Gs = b->d_Gs.p + poff[i];
+        bm.cm = b->d_cm.p + cmoff[i];
+        bm.f0 = e->d_f0.p;
+        bm.f1 = e->d_f1.p;
+        bm.n = e->nbasis;
+        bm.ns = e->nshell;
+        b->mols.push_back(bm);
+    }
+    std::vector<qf::Task> tasks;
+    std::vector<qf::TileGroup> groups;
+    auto cost = [](const qf::Task& x) { return (long long)(x.ke - x.kb) * x.ket_count; };
+    for (int cb = 0; cb < 6; ++cb)
+        for (int ck = 0; ck <= cb; ++ck)
+            for (int fp32 = 0; fp32 < 2; ++fp32)
+                for (int tile = 0; tile < 2; ++tile) {
+                    qf_engine::Launch C{};
+                    C.cb = cb;
+                    C.ck = ck;
+                    C.fp32 = fp32;
+                    C.tile = tile;
+                    C.offset = (int)tasks.size();
+                    C.goffset = (int)groups.size();
+                    C.waves = 1;
+                    for (int i = 0; i < count; ++i) {
+                        const qf_engine* e = b->engs[i];
+                        for (const auto& L : e->launches) {
+                            if (L.cb != cb || L.ck != ck || L.fp32 != fp32 || L.tile != tile) continue;
+                            const int tbase = (int)tasks.size();
+                            for (int x = L.offset; x < L.offset + L.count; ++x) {
+                                qf::Task t = e->tasks[x];
+                                t.bra_block += boff[i];
+                                t.ket_block += boff[i];
+                                tasks.push_back(t);
+                            }
+                            for (int q = L.goffset; q < L.goffset + L.gcount; ++q) {
+                                qf::TileGroup g = e->groups[q];
+                                g.task_begin += tbase - L.offset;
+                                g.task_end += tbase - L.offset;
+                                groups.push_back(g);
+                            }
+                            C.lds = std::max(C.lds, L.lds);
+                            C.waves = std::max(C.waves, L.waves);
+                        }
+                    }
+                    C.count = (int)tasks.size() - C.offset;
+                    C.gcount = (int)groups.size() - C.goffset;
+                    if (C.count == 0) continue;
+                    if (tile) {
+                        std::vector<long long> gc(C.gcount, 0);
+                        for (int q = 0; q < C.gcount; ++q)
+                            for (int x = groups[C.goffset + q].task_begin; x < groups[C.goffset + q].task_end; ++x)
+                                gc[q] += cost(tasks[x]);
+                        std::vector<int> ord(C.gcount);
+                        std::iota(ord.begin(), ord.end(), 0);
+                        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return gc[x] > gc[y]; });
+                        std::vector<qf::TileGroup> sorted(C.gcount);
+                        for (int q = 0; q < C.gcount; ++q) sorted[q] = groups[C.goffset + ord[q]];
+                        std::copy(sorted.begin(), sorted.end(), groups.begin() + C.goffset);
+                    } else {
+                        std::stable_sort(tasks.begin() + C.offset, tasks.end(),
+                                         [&](const qf::Task& x, const qf::Task& y) { return cost(x) > cost(y); });
+                    }
+                    b->launches.push_back(C);
+                }
+    if (b->d_blocks.upload(blocks) != hipSuccess || b->d_tasks.upload(tasks) != hipSuccess ||
+        b->d_groups.upload(groups) != hipSuccess || b->d_moldesc.upload(md) != hipSuccess ||
+        b->d_bm.upload(b->mols) != hipSuccess)
+        return bail(fail(QF_ERR_HIP, "batch table upload"));
+    *out = b;
+    return QF_OK;
+}
+
+int qf_batch_create(qf_engine* const* engines, int count, int replicas, qf_batch** out)
+{
+    return qf_batch_create_ext(engines, count, replicas, nullptr, nullptr, out);
+}
+
+void qf_batch_destroy(qf_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    delete b;
+}
+
+int qf_batch_size(qf_batch* b) { return b ? (int)b->mols.size() : -1; }
+
+double* qf_batch_density(qf_batch* b, int i)
+{
+    if (!b || i < 0 || i >= (int)b->mols.size()) return nullptr;
+    return const_cast<double*>(b->mols[i].P);
+}
+
+double* qf_batch_fock(qf_batch* b, int i)
+{
+    if (!b || i < 0 || i >= (int)b->mols.size()) return nullptr;
+    return b->mols[i].F;
+}
+
+const double* qf_batch_g(qf_batch* b, int i)
+{
+    if (!b || i < 0 || i >= (int)b->mols.size()) return nullptr;
+    return b->mols[i].Gs;
+}
+
+int qf_batch_set_graph(qf_batch* b, int enable)
+{
+    if (!b) return fail(QF_ERR_ARG, "NULL");
+    b->use_graph = enable != 0;
+    return QF_OK;
+}
+
+int qf_batch_build(qf_batch* b, void* stream)
+{
+    if (!b) return fail(QF_ERR_ARG, "NULL");
+    QF_HIP(hipSetDevice(b->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->use_graph || b->timing) return batch_issue(b, s);
+    if (!b->exec) {
+        // capture on a private stream (capture needs a non-legacy stream)
+        if (!b->cap_stream) QF_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        QF_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_issue(b, b->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(QF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        b->graph = g;
+        QF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    QF_HIP(hipGraphLaunch(b->exec, s));
+    return QF_OK;
+}
+
+int qf_fock_build_batch(qf_engine* const* engines, int count, const double* const* d_P, double* const* d_F,
+                        void* stream)
+{
+    // One-shot form of the batch API: a temporary batch, one launch per
+    // class for all engines.  Repeated builds should keep a qf_batch.
+    if (!engines || !d_P || count <= 0) return fail(QF_ERR_ARG, "NULL");
+    qf_batch* b = nullptr;
+    int rc = qf_batch_create(engines, count, 2, &b);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < count && !rc; ++i) {
+        const size_t n2 = (size_t)b->mols[i].n * b->mols[i].n;
+        if (hipMemcpyAsync(const_cast<double*>(b->mols[i].P), d_P[i], n2 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s) != hipSuccess)
+            rc = fail(QF_ERR_HIP, "hipMemcpyAsync (batch density)");
+    }
+    if (!rc) rc = batch_issue(b, s);
+    for (int i = 0; i < count && !rc; ++i) {
+        const int n2 = b->mols[i].n * b->mols[i].n;
+        if (d_F && d_F[i])
+            hipLaunchKernelGGL(k_add, dim3((n2 + 255) / 256), dim3(256), 0, s, d_F[i], b->mols[i].Gs, n2);
+        if (hipMemcpyAsync(engines[i]->d_Gs.p, b->mols[i].Gs, (size_t)n2 * sizeof(double), hipMemcpyDeviceToDevice,
+                           s) != hipSuccess)
+            rc = fail(QF_ERR_HIP, "hipMemcpyAsync (batch G)");
+    }
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(QF_ERR_HIP, "hipStreamSynchronize");
+    qf_batch_destroy(b);
+    return rc;
+}
+
+int qf_batch_set_density_host(qf_batch* b, int i, const double* P)
+{
+    if (!b || !P || i < 0 || i >= (int)b->mols.size()) return fail(QF_ERR_ARG, "batch density");
+    QF_HIP(hipSetDevice(b->device));
+    const size_t n2 = (size_t)b->mols[i].n * b->mols[i].n;
+    QF_HIP(hipMemcpy(const_cast<double*>(b->mols[i].P), P, n2 * sizeof(double), hipMemcpyHostToDevice));
+    return QF_OK;
+}
+
+int qf_batch_get_host(qf_batch* b, int i, double* G, double* F)
+{
+    if (!b || i < 0 || i >= (int)b->mols.size()) return fail(QF_ERR_ARG, "batch index");
+    QF_HIP(hipSetDevice(b->device));
+    const size_t n2 = (size_t)b->mols[i].n * b->mols[i].n;
+    if (G) QF_HIP(hipMemcpy(G, b->mols[i].Gs, n2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (F) QF_HIP(hipMemcpy(F, b->mols[i].F, n2 * sizeof(double), hipMemcpyDeviceToHost));
+    return QF_OK;
+}
+
+int qf_batch_clear_fock(qf_batch* b, void* stream)
+{
+    if (!b) return fail(QF_ERR_ARG, "NULL");
+    QF_HIP(hipSetDevice(b->device));
+    for (const auto& m : b->mols)
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, (hipStream_t)stream, m.F, (size_t)m.n * m.n);
+    QF_HIP(hipGetLastError());
+    return QF_OK;
+}
+
+int qf_batch_launch_count(qf_batch* b) { return b ? (int)b->launches.size() : -1; }
+
+int qf_batch_launch_info(qf_batch* b, int idx, int* cb, int* ck, int* fp32, int* ntasks)
+{
+    if (!b || idx < 0 || idx >= (int)b->launches.size()) return fail(QF_ERR_ARG, "launch index");
+    const auto& L = b->launches[idx];
+    *cb = L.cb;
+    *ck = L.ck;
+    *fp32 = L.fp32;
+    *ntasks = L.count;
+    return QF_OK;
+}
+
+int qf_batch_set_timing(qf_batch* b, int enable)
+{
+    if (!b) return fail(QF_ERR_ARG, "NULL");
+    b->timing = enable != 0;
+    b->ev_used = 0;
+    b->ev_launch.clear();
+    return QF_OK;
+}
+
+int qf_batch_read_timing(qf_batch* b, double* ms_per_launch, int* calls_per_launch)
+{
+    if (!b) return fail(QF_ERR_ARG, "NULL");
+    const size_t nl = b->launches.size();
+    for (size_t k = 0; k < nl; ++k) {
+        ms_per_launch[k] = 0.0;
+        calls_per_launch[k] = 0;
+    }
+    if (b->ev_used) QF_HIP(hipEventSynchronize(b->ev_pool[b->ev_used - 1]));
+    for (size_t r = 0; r < b->ev_launch.size(); ++r) {
+        float ms = 0.0f;
+        QF_HIP(hipEventElapsedTime(&ms, b->ev_pool[2 * r], b->ev_pool[2 * r + 1]));
+        ms_per_launch[b->ev_launch[r]] += ms;
+        calls_per_launch[b->ev_launch[r]] += 1;
+    }
+    b->ev_used = 0;
+    b->ev_launch.clear();
     return QF_OK;
 }
 

@@ -51,11 +51,11 @@ __device__ void boys_rt(double X, int maxm, int L, double (&F)[5])
     // same algorithm as qf::boys_quick, with a run-time order (L <= 4)
     double G[5];
     switch (L) {
-        case 0: { double H[1]; qf::boys_quick<0>(X, maxm, H); G[0] = H[0]; break; }
-        case 1: { double H[2]; qf::boys_quick<1>(X, maxm, H); for (int m = 0; m < 2; ++m) G[m] = H[m]; break; }
-        case 2: { double H[3]; qf::boys_quick<2>(X, maxm, H); for (int m = 0; m < 3; ++m) G[m] = H[m]; break; }
-        case 3: { double H[4]; qf::boys_quick<3>(X, maxm, H); for (int m = 0; m < 4; ++m) G[m] = H[m]; break; }
-        default: { double H[5]; qf::boys_quick<4>(X, maxm, H); for (int m = 0; m < 5; ++m) G[m] = H[m]; break; }
+        case 0: { double H[1]; qf::boys_quick<0>(X, maxm, H, qf::kBoysFit); G[0] = H[0]; break; }
+        case 1: { double H[2]; qf::boys_quick<1>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 2; ++m) G[m] = H[m]; break; }
+        case 2: { double H[3]; qf::boys_quick<2>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 3; ++m) G[m] = H[m]; break; }
+        case 3: { double H[4]; qf::boys_quick<3>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 4; ++m) G[m] = H[m]; break; }
+        default: { double H[5]; qf::boys_quick<4>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 5; ++m) G[m] = H[m]; break; }
     }
     for (int m = 0; m < 5; ++m) F[m] = m <= L ? G[m] : 0.0;
 }

@@ -19,9 +19,11 @@ namespace qf {
 #if defined(__HIP_DEVICE_COMPILE__)
 #define QF_AS_CONST __attribute__((address_space(4)))
 #define QF_AS_GLOBAL __attribute__((address_space(1)))
+#define QF_AS_LDS __attribute__((address_space(3)))
 #else
 #define QF_AS_CONST
 #define QF_AS_GLOBAL
+#define QF_AS_LDS
 #endif
 template <typename T>
 __host__ __device__ __forceinline__ const QF_AS_CONST T* cptr(const T* p)
@@ -65,52 +67,58 @@ struct VrrPar {
 // shell counts 1 even for its s part), which selects the small-X branch.
 // The fit coefficients are QUICK's; parity with QUICK needs them.
 // ---------------------------------------------------------------------------
-template <int L>
-__host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1])
+// QUICK's fits as one table: interval k (X <= 1, 1..3, 3..5, 5..10, 10..15,
+// 15..33, > 33), Horner coefficients of the fitted polynomial, highest power
+// first, padded in front with zeros (fma(0,u,0) = 0 and fma(0,u,c) = c, so
+// the padded evaluation is bit-identical to the unpadded one).  Intervals
+// 0-2: F1 = p(u), u = X, X-2, X-4, and F_0 = 2X F1 + e^-X; intervals 3-6:
+// u = 1/X and F_0 = p(u) e^-X + sqrt(pi/4/X).
+constexpr int kBoysRows = 7, kBoysCols = 12;
+__host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
+    0.0, 0.0, -8.36313918003957e-08, 1.21222603512827e-06, -1.15662609053481e-05, 9.25197374512647e-05,
+    -6.40994113129432e-04, 3.78787044215009e-03, -1.85185172458485e-02, 7.14285713298222e-02,
+    -1.99999999997023e-01, 3.33333333333318e-01,
+    -1.61702782425558e-10, 1.96215250865776e-09, -2.14234468198419e-08, 2.17216556336318e-07,
+    -1.98850171329371e-06, 1.62429321438911e-05, -1.16740298039895e-04, 7.24888732052332e-04,
+    -3.79490003707156e-03, 1.61723488664661e-02, -5.29428148329736e-02, 1.15702180856167e-01,
+    -2.62453564772299e-11, 3.24031041623823e-10, -3.614965656163e-09, 3.760256799971e-08,
+    -3.553558319675e-07, 3.022556449731e-06, -2.290098979647e-05, 1.526537461148e-04,
+    -8.81947375894379e-04, 4.33207949514611e-03, -1.75257821619926e-02, 5.28406320615584e-02,
+    0.0, 0.0, 0.0, 0.0, 0.0, 4.6897511375022e-01, -6.9955602298985e-01, 5.3689283271887e-01,
+    -3.2883030418398e-01, 2.4645596956002e-01, -4.9984072848436e-01, -3.1501078774085e-06,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, -1.8784686463512e-01, 2.2991849164985e-01,
+    -4.9893752514047e-01, -2.1916512131607e-05,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.9623264149430e-01, -4.9695241464490e-01,
+    -6.0156581186481e-05,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+// The kernels read the table from LDS: per-lane reads of a tiny table, and
+// no fit constant is held in a register across the primitive loops.
+#define QF_BOYS_LDS_DECL __shared__ double qf_boys_tab[kBoysRows * kBoysCols]
+#define QF_BOYS_LDS_FILL()                                                       \
+    do {                                                                         \
+        for (int _x = threadIdx.x; _x < kBoysRows * kBoysCols; _x += blockDim.x) \
+            qf_boys_tab[_x] = kBoysFit[_x];                                      \
+        __syncthreads();                                                         \
+    } while (0)
+#define QF_BOYS_LDS_PTR ((const QF_AS_LDS double*)qf_boys_tab)
+
+template <int L, typename Tab>
+__host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1], Tab tab)
 {
     const double XINV = 1.0 / X;
     const double E = exp(-X);
     const bool up = (X > 1.0e-1) || (X > 1.0e-4 && maxm < 4);
+    const int k = X > 5.0 ? (X > 33.0 ? 6 : X > 15.0 ? 5 : X > 10.0 ? 4 : 3) : (X > 3.0 ? 2 : X > 1.0 ? 1 : 0);
+    const double u = k >= 3 ? XINV : (k == 2 ? X - 4.0 : k == 1 ? X - 2.0 : X);
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < kBoysCols; ++c) p = fma(p, u, tab[k * kBoysCols + c]);
     double WW1;
-    if (X > 5.0) {
-        const double s = sqrt(0.78539816339744830962 * XINV);
-        if (X > 33.0) {
-            WW1 = s;
-        } else if (X > 15.0) {
-            WW1 = ((1.9623264149430e-01 * XINV - 4.9695241464490e-01) * XINV - 6.0156581186481e-05) * E + s;
-        } else if (X > 10.0) {
-            WW1 = (((-1.8784686463512e-01 * XINV + 2.2991849164985e-01) * XINV - 4.9893752514047e-01) * XINV -
-                   2.1916512131607e-05) * E + s;
-        } else {
-            WW1 = ((((((4.6897511375022e-01 * XINV - 6.9955602298985e-01) * XINV + 5.3689283271887e-01) * XINV -
-                       3.2883030418398e-01) * XINV + 2.4645596956002e-01) * XINV - 4.9984072848436e-01) * XINV -
-                   3.1501078774085e-06) * E + s;
-        }
-    } else if (X > 1.0) {
-        double F1;
-        if (X > 3.0) {
-            const double Y = X - 4.0;
-            F1 = ((((((((((-2.62453564772299e-11 * Y + 3.24031041623823e-10) * Y - 3.614965656163e-09) * Y +
-                        3.760256799971e-08) * Y - 3.553558319675e-07) * Y + 3.022556449731e-06) * Y -
-                      2.290098979647e-05) * Y + 1.526537461148e-04) * Y - 8.81947375894379e-04) * Y +
-                   4.33207949514611e-03) * Y - 1.75257821619926e-02) * Y + 5.28406320615584e-02;
-        } else {
-            const double Y = X - 2.0;
-            F1 = ((((((((((-1.61702782425558e-10 * Y + 1.96215250865776e-09) * Y - 2.14234468198419e-08) * Y +
-                        2.17216556336318e-07) * Y - 1.98850171329371e-06) * Y + 1.62429321438911e-05) * Y -
-                      1.16740298039895e-04) * Y + 7.24888732052332e-04) * Y - 3.79490003707156e-03) * Y +
-                   1.61723488664661e-02) * Y - 5.29428148329736e-02) * Y + 1.15702180856167e-01;
-        }
-        WW1 = (X + X) * F1 + E;
-    } else if (up) {
-        const double F1 = ((((((((-8.36313918003957e-08 * X + 1.21222603512827e-06) * X - 1.15662609053481e-05) * X +
-                               9.25197374512647e-05) * X - 6.40994113129432e-04) * X + 3.78787044215009e-03) * X -
-                             1.85185172458485e-02) * X + 7.14285713298222e-02) * X - 1.99999999997023e-01) * X +
-                          3.33333333333318e-01;
-        WW1 = (X + X) * F1 + E;
-    } else {
-        WW1 = (1.0 - X) / (double)(2 * maxm + 1);
-    }
+    if (k >= 3)
+        WW1 = fma(p, E, sqrt(0.78539816339744830962 * XINV));
+    else
+        WW1 = fma(X + X, p, E);
     if (up) {
         F[0] = WW1;
         const double h = 0.5 * XINV;
@@ -119,7 +127,7 @@ __host__ __device__ __forceinline__ void boys_quick(const double X, const int ma
     } else {
         // maxm exceeds L only by the s parts of SP shells (at most 4)
         const double twot = X * 2.0;
-        double f = WW1;
+        double f = (1.0 - X) / (double)(2 * maxm + 1);
 #pragma unroll
         for (int m = L + 3; m >= L; --m)
             if (m < maxm) f = (twot * f + E) * (1.0 / (2 * m + 1));
@@ -149,6 +157,12 @@ __device__ __forceinline__ void atomic_add(QF_AS_GLOBAL double* p, double v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// FP64 add into workgroup-shared memory (executed by the LDS unit).
+__device__ __forceinline__ void lds_add(QF_AS_LDS double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---------------------------------------------------------------------------
 // Contracted [e0|f0] for one (bra pair, ket pair): primitive loops with
 // QUICK's primitive screen (dnmax*cutprim(ij))*cutprim(kl) > primLimit
@@ -161,10 +175,10 @@ __device__ __forceinline__ void atomic_add(QF_AS_GLOBAL double* p, double v)
 // current one is used; the bra primitives (the same for the whole wave) are
 // the inner loop and come through scalar loads.
 // ---------------------------------------------------------------------------
-template <class C, typename R, bool BRA_UNIFORM>
+template <class C, typename R, bool BRA_UNIFORM, typename Tab>
 __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
                                          int j, const PairInfo& ki, double dn, double primlim, bool screen,
-                                         int maxm, R (&acc)[C::NACC])
+                                         int maxm, R (&acc)[C::NACC], Tab boys_tab)
 {
     const auto bp = BRA_UNIFORM ? (const double*)nullptr : bb.prim;
     const auto bpc = cptr(bb.prim);           // wave-uniform bra data
@@ -203,7 +217,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
             const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
             const double T = rpq * (zb * zk * inv);
             double F[C::L + 1];
-            boys_quick<C::L>(T, maxm, F);
+            boys_quick<C::L>(T, maxm, F, boys_tab);
             const double pref = Kb * Kk * rs;
             R Fr[C::L + 1];
 #pragma unroll
@@ -229,7 +243,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 }
 
 // ---------------------------------------------------------------------------
-// Fock kernel.  One wave per task: lanes hold up to 64 kets (c,d) of one
+// One task of the Fock build.  The 64 lanes hold up to 64 kets (c,d) of one
 // ket block; the wave loops over bras (a,b_k) of one bra block that all
 // share the first Q-shell a.  Screening per quartet as getCshellEri
 // (eri.fh:173-189).  Digestion of the canonical quartet with degeneracy
@@ -237,21 +251,31 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 //   Gt_ab += s D_cd I, Gt_cd += s D_ab I,
 //   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc I, Gt_bc -= .. D_ad I, Gt_bd -= .. D_ac I,
 // and G = Gt + Gt^T afterwards.  Because a is fixed for the wave, J_cd, K_ac
-// and K_ad stay in registers across the bra loop; J_ab is summed over the
-// wave per bra; only K_bc and K_bd go to FP64 atomics per quartet.
+// and K_ad stay in registers across the bra loop and J_ab is summed over
+// the wave per bra.
+//
+// TILE = false: everything goes straight to G with FP64 atomics (K_bc and
+// K_bd per quartet).  TILE = true: J_ab and every exchange term go to the
+// workgroup's LDS tile instead (rows: functions of a, then nb functions per
+// bra of the chunk; columns: the ket block's compact columns), and only
+// J_cd is added to G here.  FP64 atomics on HBM are executed at the memory
+// side at a fixed request rate, so the tile turns one atomic per quartet
+// into one per tile element per workgroup.
 // ---------------------------------------------------------------------------
-template <class C, typename R>
-__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
-void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ blocks,
-                 const MolDesc* __restrict__ mols)
+struct TileRef {
+    QF_AS_LDS double* k;     // nrow x ncol exchange tile
+    QF_AS_LDS double* jab;   // nbra x na x nb Coulomb terms of the bras
+    int ncol, chunk_kb;
+    const QF_AS_LDS double* boys;   // Boys fit table (kBoysFit) in LDS
+};
+
+template <class C, typename R, bool TILE>
+__device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, const PairBlock& kb,
+                                          const MolDesc& md, QF_AS_GLOBAL double* G, const TileRef& tr)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb), nc = ncart(C::Lc), nd = ncart(C::Ld);
     constexpr bool kFP32 = sizeof(R) == 4;
-    const Task t = cptr(tasks)[blockIdx.x];
-    const PairBlock bb = cptr(blocks)[t.bra_block];
-    const PairBlock kb = cptr(blocks)[t.ket_block];
-    const MolDesc md = cptr(mols)[bb.mol];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % WAVE;
     const bool in_range = lane < t.ket_count;
     const int j = in_range ? t.ket_start + lane : t.ket_start;
     const PairInfo ki = gptr(kb.info)[j];
@@ -260,8 +284,6 @@ void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ b
     const auto cm = gptr(md.cutmat);
     const auto Pc = cptr(md.P);
     const auto P = gptr(md.P);
-    // spread the atomics of concurrent waves over several copies of G
-    const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * n * n);
     const auto bperm = cptr(bb.bperm);
 
     // the shared first Q-shell a of every bra of this task
@@ -310,7 +332,7 @@ void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ b
         R acc[C::NACC];
 #pragma unroll
         for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
-        if (valid) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc);
+        if (valid) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc, tr.boys);
         R I[C::NI];
         {
             const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
@@ -356,21 +378,40 @@ void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ b
                         Kbc[b * nc + c] += Dad[a * nd + d] * v;
                         Kbd[b * nd + d] += Dac[a * nc + c] * v;
                     }
-        // bra block: summed over the wave, one atomic per element
+        // bra block: summed over the wave
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
             for (int b = 0; b < nb; ++b) {
                 const double v = wave_sum(Jab[a * nb + b]);
-                if (lane == 0 && v != 0.0) atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+                if (lane == 0 && v != 0.0) {
+                    if (TILE)
+                        lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
+                    else
+                        atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+                }
             }
         if (valid) {
+            if (TILE) {
+                const int r0 = na + (kk - tr.chunk_kb) * nb;
 #pragma unroll
-            for (int b = 0; b < nb; ++b) {
+                for (int b = 0; b < nb; ++b) {
+                    QF_AS_LDS double* row = tr.k + (r0 + b) * tr.ncol;
 #pragma unroll
-                for (int c = 0; c < nc; ++c) atomic_add(&G[(bi.fb + b) * n + ki.fa + c], kscale * Kbc[b * nc + c]);
+                    for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kbc[b * nc + c]);
 #pragma unroll
-                for (int d = 0; d < nd; ++d) atomic_add(&G[(bi.fb + b) * n + ki.fb + d], kscale * Kbd[b * nd + d]);
+                    for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kbd[b * nd + d]);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < nb; ++b) {
+#pragma unroll
+                    for (int c = 0; c < nc; ++c)
+                        atomic_add(&G[(bi.fb + b) * n + ki.fa + c], kscale * Kbc[b * nc + c]);
+#pragma unroll
+                    for (int d = 0; d < nd; ++d)
+                        atomic_add(&G[(bi.fb + b) * n + ki.fb + d], kscale * Kbd[b * nd + d]);
+                }
             }
         }
     }
@@ -379,12 +420,111 @@ void fock_kernel(const Task* __restrict__ tasks, const PairBlock* __restrict__ b
     for (int c = 0; c < nc; ++c)
 #pragma unroll
         for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
+    if (TILE) {
 #pragma unroll
-    for (int a = 0; a < na; ++a) {
+        for (int a = 0; a < na; ++a) {
+            QF_AS_LDS double* row = tr.k + a * tr.ncol;
 #pragma unroll
-        for (int c = 0; c < nc; ++c) atomic_add(&G[(fa + a) * n + ki.fa + c], kscale * Kac[a * nc + c]);
+            for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kac[a * nc + c]);
 #pragma unroll
-        for (int d = 0; d < nd; ++d) atomic_add(&G[(fa + a) * n + ki.fb + d], kscale * Kad[a * nd + d]);
+            for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kad[a * nd + d]);
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+#pragma unroll
+            for (int c = 0; c < nc; ++c) atomic_add(&G[(fa + a) * n + ki.fa + c], kscale * Kac[a * nc + c]);
+#pragma unroll
+            for (int d = 0; d < nd; ++d) atomic_add(&G[(fa + a) * n + ki.fb + d], kscale * Kad[a * nd + d]);
+        }
+    }
+}
+
+// One wave per task, every term straight to G (replica blockIdx % replicas).
+// The grid may be smaller than the task count: each workgroup then strides
+// over the tasks (a persistent launch).
+template <class C, typename R>
+__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
+void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __restrict__ blocks,
+                 const MolDesc* __restrict__ mols)
+{
+    QF_BOYS_LDS_DECL;
+    QF_BOYS_LDS_FILL();
+    TileRef tr{};
+    tr.boys = QF_BOYS_LDS_PTR;
+    for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+        const Task t = cptr(tasks)[ti];
+        const PairBlock bb = cptr(blocks)[t.bra_block];
+        const PairBlock kb = cptr(blocks)[t.ket_block];
+        const MolDesc md = cptr(mols)[bb.mol];
+        // spread the atomics of concurrent waves over several copies of G
+        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis);
+        fock_task<C, R, false>(t, bb, kb, md, G, tr);
+    }
+}
+
+// One workgroup per TileGroup (striding over the groups when the grid is
+// smaller): its waves take the group's tasks in turn and accumulate J_ab
+// and the exchange terms in LDS; after a barrier the workgroup adds the
+// non-zero tile elements to G.  Dynamic LDS:
+// (na + nbra*nb) * ncol + nbra*na*nb doubles.
+template <class C, typename R>
+__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, 4 * WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
+void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const Task* __restrict__ tasks,
+                      const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols)
+{
+    constexpr int na = ncart(C::La), nb = ncart(C::Lb);
+    extern __shared__ double qf_lds[];
+    QF_BOYS_LDS_DECL;
+    QF_BOYS_LDS_FILL();
+    QF_AS_LDS double* lds = (QF_AS_LDS double*)qf_lds;
+    const int nw = blockDim.x / WAVE;
+    for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const TileGroup g = cptr(groups)[gi];
+        const Task t0 = cptr(tasks)[g.task_begin];
+        const PairBlock bb = cptr(blocks)[t0.bra_block];
+        const PairBlock kb = cptr(blocks)[t0.ket_block];
+        const MolDesc md = cptr(mols)[bb.mol];
+        const int n = md.nbasis;
+        const int nrow = na + g.nbra * nb, ncol = kb.ncol;
+        const int ntile = nrow * ncol, njab = g.nbra * na * nb;
+        for (int x = threadIdx.x; x < ntile + njab; x += blockDim.x) lds[x] = 0.0;
+        __syncthreads();
+
+        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * n * n);
+        const TileRef tr{lds, lds + ntile, ncol, g.chunk_kb, QF_BOYS_LDS_PTR};
+        for (int ti = g.task_begin + (int)(threadIdx.x / WAVE); ti < g.task_end; ti += nw) {
+            const Task t = cptr(tasks)[ti];
+            fock_task<C, R, true>(t, bb, kb, md, G, tr);
+        }
+        __syncthreads();
+
+        // flush: row r -> function of a (r < na) or of b_k; column q -> colf[q]
+        const double kscale = -0.25 * md.hyb;
+        const auto bperm = cptr(bb.bperm);
+        const auto colf = gptr(kb.colf);
+        const int fa = cptr(bb.info)[bperm[g.chunk_kb]].fa;
+        for (int x = threadIdx.x; x < ntile; x += blockDim.x) {
+            const double v = lds[x];
+            if (v == 0.0) continue;
+            const int r = x / ncol, q = x - r * ncol;
+            int fr;
+            if (r < na) {
+                fr = fa + r;
+            } else {
+                const int k = (r - na) / nb;
+                fr = gptr(bb.info)[bperm[g.chunk_kb + k]].fb + (r - na - k * nb);
+            }
+            atomic_add(&G[fr * n + colf[q]], kscale * v);
+        }
+        for (int x = threadIdx.x; x < njab; x += blockDim.x) {
+            const double v = lds[ntile + x];
+            if (v == 0.0) continue;
+            const int k = x / (na * nb), ab = x - k * (na * nb);
+            const PairInfo bi = gptr(bb.info)[bperm[g.chunk_kb + k]];
+            atomic_add(&G[(bi.fa + ab / nb) * n + bi.fb + ab % nb], v);
+        }
+        __syncthreads();   // the next group reuses the tile
     }
 }
 
@@ -398,13 +538,15 @@ template <class C>
 __global__ __launch_bounds__(WAVE) void schwarz_kernel(PairBlock blk, double* __restrict__ out)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb);
+    QF_BOYS_LDS_DECL;
+    QF_BOYS_LDS_FILL();
     const int j = blockIdx.x * WAVE + threadIdx.x;
     if (j >= blk.n) return;
     const PairInfo pi = blk.info[j];
     double acc[C::NACC];
 #pragma unroll
     for (int k = 0; k < C::NACC; ++k) acc[k] = 0.0;
-    contract<C, double, false>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc);
+    contract<C, double, false>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc, QF_BOYS_LDS_PTR);
     double I[C::NI];
     const double AB[3] = {pi.AB[0], pi.AB[1], pi.AB[2]};
     C::template hrr<double>(acc, AB, AB, I);

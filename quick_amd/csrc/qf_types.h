@@ -23,14 +23,17 @@ struct PairInfo {
     double ycut;     // Schwarz bound of the original shell pair
     double A[3];     // centre of Q-shell a
     double AB[3];    // A - B
-    double pad;
+    int cola, colb;  // first column of Q-shells a and b in the block's compact column space
 };
 
 struct PairBlock {
     const PairInfo* info;   // n entries
     const double* prim;     // PF_N * K * n
     const int* bperm;       // pair indices grouped by first Q-shell (bra order)
+    const int* colf;        // compact column -> basis function (functions of every Q-shell in the block)
     int n, K, cls, mol;
+    int ncol;               // number of compact columns
+    int pad;
 };
 
 struct MolDesc {
@@ -50,6 +53,16 @@ struct MolDesc {
 struct Task {
     int bra_block, kb, ke, ket_block, ket_start, ket_count, same_block;
     int all_fp64;   // 1: this class runs every quartet in FP64 even in mixed mode
+};
+
+// One workgroup of the LDS-tile kernels: tasks [task_begin, task_end), all
+// from one bra chunk bperm[chunk_kb .. chunk_kb + nbra) of one bra block
+// against one ket block.  The exchange terms of the chunk are accumulated
+// in an LDS tile (rows: the functions of a and of every b_k; columns: the
+// ket block's compact columns) and added to G once, at the end.
+struct TileGroup {
+    int task_begin, task_end;
+    int chunk_kb, nbra;
 };
 
 }  // namespace qf

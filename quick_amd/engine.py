@@ -150,6 +150,17 @@ class FockEngine:
     def set_tuning(self, replicas=8, shuffle=True):
         L.check(self.lib.qf_set_tuning(self.h, int(replicas), 1 if shuffle else 0), "qf_set_tuning")
 
+    def set_tile(self, enable=True, waves=4, tasks_per_group=16, lds_bytes=64 * 1024):
+        L.check(self.lib.qf_set_tile(self.h, 1 if enable else 0, int(waves), int(tasks_per_group), int(lds_bytes)),
+                "qf_set_tile")
+
+    def set_class_tuning(self, cb, ck, tile=-1, waves=0, tasks_per_group=0, bras_per_task=0):
+        L.check(self.lib.qf_set_class_tuning(self.h, int(cb), int(ck), int(tile), int(waves), int(tasks_per_group),
+                                             int(bras_per_task)), "qf_set_class_tuning")
+
+    def set_persistent(self, max_workgroups=0):
+        L.check(self.lib.qf_set_persistent(self.h, int(max_workgroups)), "qf_set_persistent")
+
     def set_timing(self, on=True):
         L.check(self.lib.qf_set_timing(self.h, 1 if on else 0), "qf_set_timing")
 
@@ -158,4 +169,96 @@ class FockEngine:
         ms = np.zeros(max(n, 1))
         calls = np.zeros(max(n, 1), dtype=np.int32)
         L.check(self.lib.qf_read_timing(self.h, L.dptr(ms), L.iptr(calls)), "qf_read_timing")
+        return ms[:n], calls[:n]
+
+
+class FockBatch:
+    """Many molecules built together: F_i += G(P_i) for every engine with one
+    launch per quartet class (``qf_batch_*``), optionally replayed from a HIP
+    graph.  Densities and Fock accumulators live in batch-owned device slots
+    (``density_ptr(i)``, ``fock_ptr(i)``); ``build(stream)`` is asynchronous.
+    """
+
+    def __init__(self, engines, replicas=2, graph=False, density_ptrs=None, fock_ptrs=None):
+        self.lib = L.lib()
+        self.engines = list(engines)            # keeps the engines alive
+        m = len(self.engines)
+        arr = (ctypes.c_void_p * m)(*[e.h for e in self.engines])
+        h = ctypes.c_void_p()
+        if density_ptrs is None:
+            L.check(self.lib.qf_batch_create(arr, m, int(replicas), ctypes.byref(h)), "qf_batch_create")
+        else:
+            dp = (ctypes.c_void_p * m)(*density_ptrs)
+            fp = (ctypes.c_void_p * m)(*fock_ptrs)
+            L.check(self.lib.qf_batch_create_ext(arr, m, int(replicas), dp, fp, ctypes.byref(h)),
+                    "qf_batch_create_ext")
+        self.h = h
+        self.n = [e.n for e in self.engines]
+        self.set_graph(graph)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.engines)
+
+    def set_graph(self, on=True):
+        L.check(self.lib.qf_batch_set_graph(self.h, 1 if on else 0), "qf_batch_set_graph")
+
+    def density_ptr(self, i):
+        return self.lib.qf_batch_density(self.h, i)
+
+    def fock_ptr(self, i):
+        return self.lib.qf_batch_fock(self.h, i)
+
+    def g_ptr(self, i):
+        return self.lib.qf_batch_g(self.h, i)
+
+    def build(self, stream=None):
+        L.check(self.lib.qf_batch_build(self.h, ctypes.c_void_p(stream or 0)), "qf_batch_build")
+
+    def set_density(self, i, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == (self.n[i], self.n[i])
+        L.check(self.lib.qf_batch_set_density_host(self.h, i, L.dptr(P)), "qf_batch_set_density_host")
+
+    def get(self, i, g=True, f=False):
+        """(G_i, F_i) as host arrays (synchronous)."""
+        n = self.n[i]
+        G = np.zeros((n, n)) if g else None
+        F = np.zeros((n, n)) if f else None
+        null = ctypes.POINTER(ctypes.c_double)()
+        L.check(self.lib.qf_batch_get_host(self.h, i, L.dptr(G) if g else null, L.dptr(F) if f else null),
+                "qf_batch_get_host")
+        return G, F
+
+    def clear_fock(self, stream=None):
+        L.check(self.lib.qf_batch_clear_fock(self.h, ctypes.c_void_p(stream or 0)), "qf_batch_clear_fock")
+
+    def launches(self):
+        out = []
+        for k in range(self.lib.qf_batch_launch_count(self.h)):
+            cb, ck, f32, nt = (ctypes.c_int() for _ in range(4))
+            L.check(self.lib.qf_batch_launch_info(self.h, k, ctypes.byref(cb), ctypes.byref(ck), ctypes.byref(f32),
+                                                  ctypes.byref(nt)), "qf_batch_launch_info")
+            out.append(("(%s|%s)" % (FockEngine.CLASS_NAMES[cb.value], FockEngine.CLASS_NAMES[ck.value]),
+                        bool(f32.value), nt.value))
+        return out
+
+    def set_timing(self, on=True):
+        L.check(self.lib.qf_batch_set_timing(self.h, 1 if on else 0), "qf_batch_set_timing")
+
+    def read_timing(self):
+        n = self.lib.qf_batch_launch_count(self.h)
+        ms = np.zeros(max(n, 1))
+        calls = np.zeros(max(n, 1), dtype=np.int32)
+        L.check(self.lib.qf_batch_read_timing(self.h, L.dptr(ms), L.iptr(calls)), "qf_batch_read_timing")
         return ms[:n], calls[:n]

@@ -28,7 +28,9 @@ def declared_functions():
 
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build with python -m quick_amd.build"
-    ctypes.CDLL(LIB)
+    from quick_amd import _lib
+    assert os.path.samefile(_lib.LIBPATH, LIB)
+    _lib.lib()
     out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True).stdout
     exported = set(l.split()[-1] for l in out.splitlines() if " T " in l)
     decl = declared_functions()
