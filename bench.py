@@ -47,12 +47,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--cpu-builds", type=int, default=5, help="oracle builds timed for the CPU baseline (median)")
+    ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10"],
+                    help="h2o32: (H2O)32 6-31G(d) builds (default); batch: 256 uneven molecules per build "
+                         "(plain FP64 vs batched graph FP64 vs batched graph mixed); serve: Poisson SCF "
+                         "requests through the continuous-batching server; gly10: mixed-precision SCF error")
+    ap.add_argument("--molecules", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
+    ap.add_argument("--autotune", type=int, default=1, help="tune launch shapes per class before timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus %d is not supported "
-                                   "(the multi-GPU design is described in DESIGN.md)" % a.gpus}))
+        print(json.dumps({"error": "this build measures one GPU; --gpus %d is not supported "
+                                   "(sharding over several GPUs is a design only)" % a.gpus}))
         sys.exit(2)
 
     import torch
@@ -63,6 +71,12 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.workload == "batch":
+        return run_batch(a, dev)
+    if a.workload == "serve":
+        return run_serve(a, dev)
+    if a.workload == "gly10":
+        return run_gly10(a, dev)
     if a.copy_probe:
         nbytes = a.copy_probe << 20
         x = torch.ones(nbytes, dtype=torch.uint8, device=dev)
@@ -103,6 +117,14 @@ def main():
     F = torch.zeros((n, n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
+    tuned = None
+    if a.autotune:
+        from quick_amd.autotune import autotune
+        ta = time.perf_counter()
+        tuned = autotune(eng, sh, builds=2, make_density=lambda i: P.data_ptr())
+        torch.cuda.synchronize()
+        log(f"[bench] autotune {time.perf_counter() - ta:.2f}s: " +
+            ", ".join(f"{k}:{v[0]}" for k, v in sorted(tuned.items())))
     for _ in range(a.warmup):
         eng.fock2e_device(P.data_ptr(), F.data_ptr(), sh)
     torch.cuda.synchronize()
@@ -192,7 +214,157 @@ def main():
         "build_gflop": round(build_flops / 1e9, 3),
         "wall_s_timed": round(wall, 3),
         "scf": scf,
+        "autotuned": {k: list(v[0]) for k, v in sorted(tuned.items())} if tuned else None,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_batch(a, dev):
+    """256 molecules of uneven size (water clusters 1-16, n-alkanes C1-C12),
+    6-31G(d): all Fock builds of one SCF iteration.  Plain FP64 (one
+    molecule after another) vs one batched build per iteration replayed from
+    a HIP graph, in FP64 and in mixed precision."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.autotune import autotune
+    from quick_amd.engine import FockBatch, FockEngine
+    from quick_amd.scf import seeded_density
+    mols = M.mixed_set(a.molecules)
+    t0 = time.perf_counter()
+    e64, emx, dens = [], [], []
+    for m in mols:
+        bs = B.build_basis(m, "6-31G*")
+        e = FockEngine(bs, cutoff=1e-10)
+        S, _, _ = e.one_electron()
+        dens.append(seeded_density(S, m.nelec() // 2))
+        e64.append(e)
+        emx.append(FockEngine(bs, cutoff=1e-10, precision="mixed"))
+    nb = [e.n for e in e64]
+    log(f"[batch] {len(mols)} molecules, nbasis {min(nb)}..{max(nb)}, setup {time.perf_counter() - t0:.1f}s")
+    s = torch.cuda.current_stream(dev).cuda_stream
+    # plain FP64: every molecule on its own, as a per-molecule drop-in would
+    Pd = [torch.from_numpy(P).to(dev) for P in dens]
+    Fd = [torch.zeros_like(p) for p in Pd]
+
+    def plain():
+        for e, p, f in zip(e64, Pd, Fd):
+            e.fock2e_device(p.data_ptr(), f.data_ptr(), s)
+    t_plain = _timed(plain, max(1, a.steps // 4), 1)
+    res = {}
+    for name, engs in (("fp64", e64), ("mixed", emx)):
+        tuned = None
+        if a.autotune:
+            tuned = autotune(engs, s, builds=1, batch_kw={"replicas": 2}, densities=dens)
+        b = FockBatch(engs, replicas=2, graph=True)
+        for i, P in enumerate(dens):
+            b.set_density(i, P)
+        t = _timed(lambda: b.build(s), a.steps, a.warmup)
+        res[name] = (t, b, tuned)
+    # accuracy of the mixed build against the FP64 build, per molecule
+    dF, dE = 0.0, 0.0
+    b64, bmx = res["fp64"][1], res["mixed"][1]
+    for i, P in enumerate(dens):
+        G64, _ = b64.get(i)
+        Gmx, _ = bmx.get(i)
+        dF = max(dF, float(np.max(np.abs(Gmx - G64))))
+        dE = max(dE, 0.5 * abs(float(np.sum(P * (Gmx - G64)))))
+    nm = len(mols)
+    out = {
+        "metric": METRIC,
+        "value": round(nm / res["mixed"][0], 3),
+        "unit": "Fock builds/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(res["mixed"][0] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{nm} molecules (water clusters 1-16, n-alkanes C1-C12) RHF/6-31G(d), "
+                               "all Fock builds of one SCF iteration per step, batched HIP-graph replay",
+                   "nbasis_min": min(nb), "nbasis_max": max(nb), "nbasis_sum": int(sum(nb)),
+                   "integral_cutoff": 1e-10, "precision": "mixed (split 1e-4)"},
+        "plain_fp64_builds_per_s": round(nm / t_plain, 3),
+        "batched_fp64_builds_per_s": round(nm / res["fp64"][0], 3),
+        "batched_mixed_builds_per_s": round(nm / res["mixed"][0], 3),
+        "speedup_mixed_batched_vs_plain_fp64": round(t_plain / res["mixed"][0], 3),
+        "mixed_max_abs_dF_eh": dF, "mixed_max_abs_dE2_eh": dE,
+        "autotuned": {k: {n: list(v[0]) for n, v in sorted(r[2].items())} for k, r in res.items() if r[2]},
+    }
+    for r in res.values():
+        r[1].close()
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(a, dev):
+    """Poisson SCF requests (molecules of 3-60 atoms from the batch generator,
+    basis uniform over 6-31G / 6-31G(d) / def2-SVP, mixed precision) through
+    the continuous-batching server: requests/s and p50/p99 latency."""
+    from quick_amd.serve import FockServer, request_stream, summarize
+    reqs = request_stream(a.requests, a.rate)
+    srv = FockServer(dev, precision="mixed")
+    # one warm-up request (kernel loading, allocator)
+    srv.run(request_stream(1, 1000.0, seed=7))
+    results, st = srv.run(reqs, log=None)
+    sm = summarize(results, st["wall_s"])
+    out = {
+        "metric": METRIC, "value": round(sm["requests_per_s"], 4), "unit": "SCF requests/s",
+        "n_gpus": 1, "steps": st["steps"], "warmup": 1, "ms_per_step": round(1e3 * st["wall_s"] / max(1, st["steps"]), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{a.requests} Poisson RHF SCF requests at {a.rate}/s, molecules 3-60 atoms, "
+                               "basis in {6-31G, 6-31G(d), def2-SVP}, mixed precision, continuous batching",
+                   "rate_per_s": a.rate},
+        "p50_latency_s": round(sm["p50_s"], 4), "p99_latency_s": round(sm["p99_s"], 4),
+        "converged": sm["converged"], "wall_s": round(st["wall_s"], 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_gly10(a, dev):
+    """(Gly)10 6-31G(d,p): SCF in FP64 and in mixed precision (FP32 VRR/HRR
+    for quartets with Ycut*Ycut < 1e-4 max), dF of one build and dE_SCF."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.engine import FockEngine
+    from quick_amd.scf import rhf
+    mol = M.polyglycine(10)
+    bs = B.build_basis(mol, "6-31G**")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        e = FockEngine(bs, cutoff=1e-11, prim_limit=1e-12, precision=prec)
+        S, T, V = e.one_electron()
+        r = rhf(e, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
+        P = torch.from_numpy(r.density).to(dev)
+        F = torch.zeros_like(P)
+        sh = torch.cuda.current_stream(dev).cuda_stream
+        t = _timed(lambda: e.fock2e_device(P.data_ptr(), F.data_ptr(), sh), a.steps, a.warmup)
+        res[prec] = (r, t, e)
+    P64 = res["fp64"][0].density
+    G64 = res["fp64"][2].fock2e(P64)
+    Gmx = res["mixed"][2].fock2e(P64)
+    out = {
+        "metric": METRIC, "value": round(1.0 / res["mixed"][1], 3), "unit": "Fock builds/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(res["mixed"][1] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "(Gly)10 beta strand RHF/6-31G(d,p), split-precision Fock build", "natom": mol.natom,
+                   "nbasis": bs.nbasis, "integral_cutoff": 1e-11, "split": 1e-4},
+        "fp64_builds_per_s": round(1.0 / res["fp64"][1], 3),
+        "scf_energy_fp64": res["fp64"][0].energy, "scf_energy_mixed": res["mixed"][0].energy,
+        "dE_scf_eh": abs(res["mixed"][0].energy - res["fp64"][0].energy),
+        "max_abs_dF_eh": float(np.max(np.abs(Gmx - G64))),
+        "scf_iterations": [res["fp64"][0].iterations, res["mixed"][0].iterations],
     }
     print(json.dumps(out), flush=True)
 

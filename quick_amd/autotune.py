@@ -41,11 +41,11 @@ def _class_index(name):
     return names.index(a), names.index(b)
 
 
-def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None):
+def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None, densities=None):
     """Choose a launch shape per class.  ``engines``: one FockEngine or a
-    list (tuned as one FockBatch).  ``make_density(i)`` returns a device
-    pointer (int) of a density for engine i (single engine) or is ignored
-    for batches (their density slots are used as they are).  Returns
+    list (tuned as one FockBatch).  Single engine: ``make_density(0)``
+    returns the device pointer (int) of the density to build with.  Batch:
+    ``densities`` holds one host density per engine.  Returns
     {class name: (candidate, ms)} and leaves the best shapes applied."""
     from .engine import FockBatch
     single = not isinstance(engines, (list, tuple))
@@ -63,6 +63,8 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
                                        e.set_timing, builds)
         else:
             b = FockBatch(engs, **(batch_kw or {}))
+            for i, P in enumerate(densities):
+                b.set_density(i, P)
             times[cand] = _class_times(lambda: b.build(stream), b.launches, b.read_timing, b.set_timing, builds)
             b.close()
     # per class the candidate with the smallest time summed over its FP64 and

@@ -587,12 +587,6 @@ int prepare(qf_engine* e)
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32) {
-                // (dd|dp) and (dd|dd) stay FP64 in mixed mode: their float
-                // kernels spill several KB per lane (little to gain) and the
-                // (dd|dp) one returned non-finite values on gfx950.
-                const bool split_class = e->prec == QF_PREC_MIXED &&
-                                         kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
-                if (fp32 && !split_class) continue;
                 const int offset = (int)e->tasks.size();
                 const int goffset = (int)e->groups.size();
                 const int na = ncart(kClassL[cb][0]), nb = ncart(kClassL[cb][1]);
@@ -611,7 +605,16 @@ int prepare(qf_engine* e)
                     if (Kb.cls == ck) maxcol = std::max(maxcol, (int)Kb.colf.size());
                 const long long room = e->lds_budget / 8 - (long long)na * maxcol;
                 const long long mfit = room > 0 ? room / ((long long)nb * maxcol + na * nb) : 0;
-                const bool tile = (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0) && mfit >= 1;
+                // Mixed precision splits a class only where its FP32 half can run
+                // through the LDS-tile kernel: the direct-atomic FP32 kernels of
+                // (dp|dp) gave wrong Fock matrices on gfx950 and the (dd|dp) one
+                // non-finite values, while CPU builds of the same code are right.
+                // (dd|dp) and (dd|dd) stay FP64 (their FP32 forms spill several KB
+                // per lane: little to gain).
+                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 &&
+                                         kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
+                if (fp32 && !split_class) continue;
+                const bool tile = (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
                 size_t lds = 0;
                 long long group_tasks = 0;

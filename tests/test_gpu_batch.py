@@ -96,7 +96,7 @@ def test_one_shot_batch_api_and_autotune_keep_results():
         assert np.max(np.abs(Fd[i].cpu().numpy() - singles[i])) <= 1e-12
     # autotune over the batch, then the single builds with the tuned shapes
     s = torch.cuda.current_stream().cuda_stream
-    chosen = autotune(engs, s, builds=1)
+    chosen = autotune(engs, s, builds=1, densities=dens)
     assert chosen
     for e, P, G1 in zip(engs, dens, singles):
         assert np.max(np.abs(e.fock2e(P) - G1)) <= 1e-12

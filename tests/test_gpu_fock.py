@@ -163,6 +163,26 @@ def test_mixed_precision_error_is_bounded():
     assert dE < 1e-6
 
 
+@pytest.mark.parametrize("name", ["water3", "alkane3"])
+def test_mixed_precision_with_direct_fp64_launches(name):
+    """Tile digestion off: the FP64 halves run through the direct-atomic
+    kernels while every FP32 half still runs through the LDS-tile kernels."""
+    announce(f"mixed precision, direct FP64 launches, {name} 6-31G*")
+    mol = molecule(name)
+    bs = product_basis(mol, "6-31G*")
+    e64 = _engine(bs, cutoff=1e-10)
+    S, _, _ = e64.one_electron()
+    from quick_amd.scf import seeded_density
+    P = seeded_density(S, mol.nelec() // 2)
+    G64 = e64.fock2e(P)
+    emx = _engine(bs, cutoff=1e-10, precision="mixed")
+    emx.set_tile(False)
+    dF = float(np.max(np.abs(emx.fock2e(P) - G64)))
+    assert emx.stats()["n_tasks_fp32"] > 0
+    print(f"  max|dF| {dF:.2e}", flush=True)
+    assert dF < 1e-6
+
+
 def test_quick_fortran_abi_drop_in():
     """Call the gpu_*_ entry points the way QUICK's Fortran does: 1-based
     shell arrays, gccoeff/gcexpo(MAXPRIM=14, nbasis) column-major,
