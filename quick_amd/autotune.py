@@ -11,8 +11,12 @@ from __future__ import annotations
 
 import numpy as np
 
-# (tile, waves, tasks_per_group, bras_per_task); 0 = engine default
+# (digestion, waves, tasks_per_group, bras_per_task); digestion 0 direct
+# atomics, 1 LDS tile, 2 wave per quartet (classes without that kernel use
+# the tile); 0 elsewhere = engine default
 CANDIDATES = [
+    (2, 0, 0, 0),
+    (2, 0, 0, 4),
     (0, 0, 0, 0),
     (1, 4, 16, 0),
     (1, 4, 8, 0),

@@ -40,11 +40,12 @@ def sources():
 
 
 def _deps(src):
-    deps = [src, os.path.join(CSRC, "qf_types.h"), os.path.join(CSRC, "qf_device.h"),
+    deps = [src, os.path.join(CSRC, "qf_types.h"), os.path.join(CSRC, "qf_device.h"), os.path.join(CSRC, "qf_coop.h"),
             os.path.join(os.path.dirname(PKG), "include", "quickfock.h"), os.path.join(CSRC, "gen", "launchers.h")]
     base = os.path.basename(src)
     if base.startswith("eri_"):
         deps.append(os.path.join(CSRC, "gen", base[4:-4] + ".h"))
+        deps.append(os.path.join(CSRC, "gen", "coop_" + base[4:-4] + ".h"))
     return deps
 
 

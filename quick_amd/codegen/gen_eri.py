@@ -250,6 +250,148 @@ class Hrr:
         return lines, flops
 
 
+# Classes that also get a wave-per-quartet kernel (fock_coop_kernel): the
+# VRR and HRR run as level-scheduled node programs over LDS, so a quartet's
+# intermediates never leave the chip.
+COOP_CLASSES = [(1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0),
+                (2, 2, 1, 0), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+
+
+def coop_tables(La, Lb, Lc, Ld):
+    """Node programs of the wave-per-quartet kernel for one class.
+
+    LDS slots (doubles): [0, NACC) contracted [e0|f0] accumulators; from NACC
+    a region shared by the VRR (base [00|00]^(m), m = 0..L, the VRR nodes in
+    level order, one zero slot) and afterwards the HRR (ket nodes, then bra
+    nodes, each in level order); then the coefficient table (per (bra|ket,
+    direction): PA|QC, WP|WQ, 1/2z|1/2n, rho/z|rho/n; 1/2(z+n); CD and AB),
+    the integrals with cons and degeneracy, and the six density blocks."""
+    v = Vrr(La, Lb, Lc, Ld)
+    L = v.L
+    acc = {t: i for i, t in enumerate(v.targets)}
+    nacc = len(v.targets)
+    order = [k for k in v.order() if v.nodes[k][0] != "base"]
+    nv = len(order)
+    slot = {}
+    for m in range(L + 1):
+        slot[(Z, Z, m)] = nacc + m
+    for idx, k in enumerate(order):
+        slot[k] = nacc + L + 1 + idx
+    zs = nacc + L + 1 + nv
+    vnodes, vlev = [], [0]
+    cur = None
+    for idx, k in enumerate(order):
+        lv = am(k[0]) + am(k[1])
+        if cur is not None and lv != cur:
+            vlev.append(idx)
+        cur = lv
+        kind, (i, deps) = v.nodes[k]
+        a, b = slot[deps[0][2]], slot[deps[1][2]]
+        c = d = e = zs
+        n1 = n2 = 0
+        if kind == "bra":
+            sel = i
+            if len(deps) > 2:
+                n1, c, d = deps[2][1], slot[deps[2][2]], slot[deps[2][3]]
+        else:
+            sel = 3 + i
+            for dd in deps[2:]:
+                if dd[0] == "N2":
+                    n1, c, d = dd[1], slot[dd[2]], slot[dd[3]]
+                else:
+                    n2, e = dd[1], slot[dd[2]]
+        vnodes.append((a, b, c, d, e, slot[k], sel, n1, n2))
+    vlev.append(nv)
+    tgt = [slot[(e, f, 0)] for (e, f) in v.targets]
+    # HRR node programs
+    h = Hrr(La, Lb, Lc, Ld)
+    outs = []
+    for a in comps(La):
+        for b in comps(Lb):
+            for c in comps(Lc):
+                for d in comps(Ld):
+                    h.need_bra(a, b, c, d)
+                    outs.append((a, b, c, d))
+    korder = sorted(h.order_k, key=lambda k: am(k[2]))
+    border = sorted(h.order_b, key=lambda k: am(k[1]))
+    nk, nb_ = len(korder), len(border)
+    kslot = {k: nacc + i for i, k in enumerate(korder)}
+    bslot = {k: nacc + nk + i for i, k in enumerate(border)}
+
+    def ks(e, c, d):
+        return acc[(e, c)] if d == Z else kslot[(e, c, d)]
+
+    def bs(a, b, c, d):
+        return ks(a, c, d) if b == Z else bslot[(a, b, c, d)]
+
+    hnodes, hlev = [], [0]
+    cur = None
+    for idx, (e, c, d) in enumerate(korder):
+        lv = ("k", am(d))
+        if cur is not None and lv != cur:
+            hlev.append(idx)
+        cur = lv
+        i = h.ket[(e, c, d)]
+        hnodes.append((kslot[(e, c, d)], ks(e, add(c, i), sub(d, i)), ks(e, c, sub(d, i)), i))
+    for idx, (a, b, c, d) in enumerate(border):
+        lv = ("b", am(b))
+        if cur is not None and lv != cur:
+            hlev.append(nk + idx)
+        cur = lv
+        i = h.bra[(a, b, c, d)]
+        hnodes.append((bslot[(a, b, c, d)], bs(add(a, i), sub(b, i), c, d), bs(a, sub(b, i), c, d), 3 + i))
+    hlev.append(nk + nb_)
+    omap = [bs(*o) for o in outs]
+    region = max(L + 2 + nv, nk + nb_)
+    na, nb, nc, nd = (len(comps(x)) for x in (La, Lb, Lc, Ld))
+    ni = na * nb * nc * nd
+    cf = nacc + region
+    ic = cf + 32
+    dm = ic + ni
+    total = dm + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd
+    assert total * 8 <= 64 * 1024, (La, Lb, Lc, Ld, total)
+    assert all(max(x[:6]) < 65536 for x in vnodes) and all(max(x[:3]) < 65536 for x in hnodes)
+    return dict(L=L, nacc=nacc, nv=nv, vlev=vlev, vnodes=vnodes, tgt=tgt, zs=zs, nk=nk, nb=nb_, hnodes=hnodes,
+                hlev=hlev, omap=omap, ni=ni, cf=cf, ic=ic, dm=dm, total=total, outs=outs, targets=v.targets)
+
+
+def coop_source(name, La, Lb, Lc, Ld):
+    t = coop_tables(La, Lb, Lc, Ld)
+    L, nacc, nv, vlev, vnodes, tgt, zs = t["L"], t["nacc"], t["nv"], t["vlev"], t["vnodes"], t["tgt"], t["zs"]
+    nk, nb_, hnodes, hlev, omap, ni = t["nk"], t["nb"], t["hnodes"], t["hlev"], t["omap"], t["ni"]
+    cf, ic, dm, total = t["cf"], t["ic"], t["dm"], t["total"]
+    cname = name + "Coop"
+    src = ["struct %s {" % cname,
+           "  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L),
+           "  static constexpr int NACC = %d, NV = %d, NLV = %d, NH = %d, NLH = %d, NI = %d;"
+           % (nacc, nv, len(vlev) - 1, nk + nb_, len(hlev) - 1, ni),
+           "  static constexpr int BASE = %d, ZS = %d, CF = %d, IC = %d, DM = %d, LDS = %d;"
+           % (nacc, zs, cf, ic, dm, total),
+           "  static constexpr int NT = %d;   // threads per workgroup" % (256 if nv >= 1000 else 64),
+           "  __device__ static const CoopVNode* vrr();",
+           "  __device__ static const int* vlev();",
+           "  __device__ static const unsigned short* tgt();",
+           "  __device__ static const CoopHNode* hrr();",
+           "  __device__ static const int* hlev();",
+           "  __device__ static const unsigned short* out();",
+           "};"]
+    fmt_v = ",".join("{%d,%d,%d,%d,%d,%d,%d,%d,%d,0}" % x for x in vnodes)
+    fmt_h = ",".join("{%d,%d,%d,%d,0}" % x for x in hnodes)
+    src += ["__device__ const CoopVNode k%s_vrr[%d] = {%s};" % (cname, nv, fmt_v),
+            "__device__ const int k%s_vlev[%d] = {%s};" % (cname, len(vlev), ",".join(map(str, vlev))),
+            "__device__ const unsigned short k%s_tgt[%d] = {%s};" % (cname, nacc, ",".join(map(str, tgt))),
+            "__device__ const CoopHNode k%s_hrr[%d] = {%s};" % (cname, max(1, nk + nb_), fmt_h or "{0,0,0,0,0}"),
+            "__device__ const int k%s_hlev[%d] = {%s};" % (cname, len(hlev), ",".join(map(str, hlev))),
+            "__device__ const unsigned short k%s_out[%d] = {%s};" % (cname, ni, ",".join(map(str, omap))),
+            "__device__ inline const CoopVNode* %s::vrr() { return k%s_vrr; }" % (cname, cname),
+            "__device__ inline const int* %s::vlev() { return k%s_vlev; }" % (cname, cname),
+            "__device__ inline const unsigned short* %s::tgt() { return k%s_tgt; }" % (cname, cname),
+            "__device__ inline const CoopHNode* %s::hrr() { return k%s_hrr; }" % (cname, cname),
+            "__device__ inline const int* %s::hlev() { return k%s_hlev; }" % (cname, cname),
+            "__device__ inline const unsigned short* %s::out() { return k%s_out; }" % (cname, cname)]
+    return cname, "\n".join(src) + "\n", total
+
+
 PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
 
 
@@ -322,9 +464,9 @@ def gen_class(La, Lb, Lc, Ld):
                                              vrr_flop=vflops, hrr_flop=hflops, nodes=len(vrr.nodes))
 
 
-def launcher_source(name, diag):
+def launcher_source(name, diag, coop=False):
     out = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.",
-           '#include "%s.h"' % name, "", "namespace qf {", "",
+           '#include "%s.h"' % name] + (['#include "coop_%s.h"' % name] if coop else []) + ["", "namespace qf {", "",
            "void launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
            " int grid, hipStream_t s)" % name, "{",
            "    if (nt <= 0) return;",
@@ -345,6 +487,14 @@ def launcher_source(name, diag):
            "        hipLaunchKernelGGL((fock_tile_kernel<%s, double>), dim3(grid), dim3(waves * WAVE), lds, s,"
            " g, ng, t, b, m);" % name,
            "}", ""]
+    if coop:
+        out += ["void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int grid,"
+                " hipStream_t s)" % name, "{",
+                "    if (nt <= 0) return;",
+                "    if (grid <= 0 || grid > nt) grid = nt;",
+                "    hipLaunchKernelGGL((fock_coop_kernel<%sCoop>), dim3(grid), dim3(%sCoop::NT), %sCoop::LDS * sizeof(double),"
+                " s, t, nt, b, m);" % (name, name, name),
+                "}", ""]
     if diag:
         out += ["void launch_schwarz_%s(PairBlock blk, double* out, hipStream_t s)" % name, "{",
                 "    if (blk.n <= 0) return;",
@@ -375,7 +525,13 @@ def main():
                "#pragma once\n#include \"../qf_device.h\"\nnamespace qf {\n" + src + "}  // namespace qf\n")
         _write(os.path.join(OUT, name + ".h"), hdr)
         diag = (La, Lb) == (Lc, Ld)
-        _write(os.path.join(OUT, "eri_" + name + ".hip"), launcher_source(name, diag))
+        coop = (La, Lb, Lc, Ld) in COOP_CLASSES
+        if coop:
+            cname, csrc, ctotal = coop_source(name, La, Lb, Lc, Ld)
+            _write(os.path.join(OUT, "coop_" + name + ".h"),
+                   "// Generated by quick_amd/codegen/gen_eri.py - do not edit.\n#pragma once\n"
+                   "#include \"../qf_coop.h\"\nnamespace qf {\n" + csrc + "}  // namespace qf\n")
+        _write(os.path.join(OUT, "eri_" + name + ".hip"), launcher_source(name, diag, coop))
         names.append((name, La, Lb, Lc, Ld, diag))
         info["boys_setup_flop"] = boys_setup_flop(La + Lb + Lc + Ld)
         info["prim_flop"] = info["vrr_flop"] + info["boys_setup_flop"]
@@ -388,12 +544,16 @@ def main():
              "using FockLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
              "using FockTileLaunch = void (*)(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
              " int, int, size_t, int, hipStream_t);",
+             "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);",
              "using SchwarzLaunch = void (*)(PairBlock, double*, hipStream_t);", ""]
     for (name, La, Lb, Lc, Ld, diag) in names:
         lines.append("void launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
                      % name)
         lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
                      " int, int, size_t, int, hipStream_t);" % name)
+        if (La, Lb, Lc, Ld) in COOP_CLASSES:
+            lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);"
+                         % name)
         if diag:
             lines.append("void launch_schwarz_%s(PairBlock, double*, hipStream_t);" % name)
     # FLOP model (FMA = 2): per surviving primitive quartet the VRR plus
@@ -416,6 +576,11 @@ def main():
     lines += ["    return nullptr;", "}", "", "inline FockTileLaunch fock_tile_launcher(int cb, int ck)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
         lines.append("    if (cb == %d && ck == %d) return launch_fock_tile_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
+    lines += ["    return nullptr;", "}", "", "// wave-per-quartet kernels (nullptr: the class has none)",
+              "inline FockCoopLaunch fock_coop_launcher(int cb, int ck)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        if (La, Lb, Lc, Ld) in COOP_CLASSES:
+            lines.append("    if (cb == %d && ck == %d) return launch_fock_coop_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
     lines += ["    return nullptr;", "}", "", "inline SchwarzLaunch schwarz_launcher(int c)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
         if diag:

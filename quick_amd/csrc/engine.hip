@@ -44,6 +44,7 @@ constexpr double kPi = 3.14159265358979323846264338327950288;
 const double kX0 = 2.0 * std::pow(kPi, 2.5);   // QUICK's X0 (quick_constants_module.f90:61)
 
 inline int pair_class(int La, int Lb) { return La * (La + 1) / 2 + Lb; }
+constexpr int kCoopWindow = 8;   // kets per task of the wave-per-quartet kernels
 const int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
@@ -614,8 +615,11 @@ int prepare(qf_engine* e)
                 const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 &&
                                          kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
                 if (fp32 && !split_class) continue;
-                const bool tile = (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
+                // wave-per-quartet kernel (FP64 launches of classes that have one)
+                const bool coop = !fp32 && ct.tile == 2 && qf::fock_coop_launcher(cb, ck) != nullptr;
+                const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
+                const int win = coop ? kCoopWindow : qf::WAVE;
                 size_t lds = 0;
                 long long group_tasks = 0;
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {
@@ -648,10 +652,10 @@ int prepare(qf_engine* e)
                                 nmax = std::max(nmax, hi[k]);
                             }
                             const int chunk_first = (int)e->tasks.size();
-                            for (int s0 = 0; s0 < nmax; s0 += qf::WAVE) {
+                            for (int s0 = 0; s0 < nmax; s0 += win) {
                                 int kb = -1, ke = -1;
                                 for (int k = 0; k < M; ++k)
-                                    if (lo[k] < hi[k] && lo[k] < s0 + qf::WAVE && hi[k] > s0) {
+                                    if (lo[k] < hi[k] && lo[k] < s0 + win && hi[k] > s0) {
                                         if (kb < 0) kb = k;
                                         ke = k + 1;
                                     }
@@ -662,7 +666,7 @@ int prepare(qf_engine* e)
                                 t.ke = g.first + ke;
                                 t.ket_block = (int)kbk;
                                 t.ket_start = s0;
-                                t.ket_count = std::min(qf::WAVE, nmax - s0);
+                                t.ket_count = std::min(win, nmax - s0);
                                 t.same_block = same ? 1 : 0;
                                 t.all_fp64 = split_class ? 0 : 1;
                                 e->tasks.push_back(t);
@@ -722,7 +726,7 @@ int prepare(qf_engine* e)
                     L.fp32 = fp32;
                     L.offset = offset;
                     L.count = count;
-                    L.tile = tile && gcount > 0;
+                    L.tile = coop ? 2 : (tile && gcount > 0) ? 1 : 0;
                     L.goffset = goffset;
                     L.gcount = gcount;
                     L.waves = gcount > 0 ? (int)std::min<long long>(tile_waves,
@@ -735,7 +739,7 @@ int prepare(qf_engine* e)
             }
     // every tile workgroup must stay inside its chunk and its LDS allocation
     for (const auto& L : e->launches) {
-        if (!L.tile) continue;
+        if (L.tile != 1) continue;
         const int na = ncart(kClassL[L.cb][0]), nb = ncart(kClassL[L.cb][1]);
         for (int q = L.goffset; q < L.goffset + L.gcount; ++q) {
             const qf::TileGroup& g = e->groups[q];
@@ -820,7 +824,11 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
             e->ev_launch.push_back((int)li);
             QF_HIP(hipEventRecord(ea, s));
         }
-        if (L.tile) {
+        if (L.tile == 2) {
+            qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
+            if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
+            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, e->persist, s);
+        } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
             ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, e->d_mol.p, L.fp32, L.waves, L.lds,
@@ -1114,7 +1122,8 @@ int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int t
     if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
     if (waves > 4 || bras_per_task > 64) return fail(QF_ERR_ARG, "waves <= 4 and bras_per_task <= 64");
     qf_engine::ClassTune& t = e->ctune[cb][ck];
-    t.tile = tile < 0 ? -1 : (tile != 0);
+    if (tile > 2) return fail(QF_ERR_ARG, "tile must be -1, 0 (direct), 1 (LDS tile) or 2 (wave per quartet)");
+    t.tile = tile < 0 ? -1 : tile;
     t.waves = std::max(0, waves);
     t.tpg = std::max(0, tasks_per_group);
     t.mmax = std::max(0, bras_per_task);
@@ -1272,7 +1281,11 @@ int batch_issue(qf_batch* b, hipStream_t s)
             b->ev_launch.push_back((int)li);
             QF_HIP(hipEventRecord(ea, s));
         }
-        if (L.tile) {
+        if (L.tile == 2) {
+            qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
+            if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
+            fc(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, b->persist, s);
+        } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
             ft(b->d_groups.p + L.goffset, L.gcount, b->d_tasks.p, b->d_blocks.p, b->d_moldesc.p, L.fp32, L.waves,
@@ -1381,7 +1394,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32)
-                for (int tile = 0; tile < 2; ++tile) {
+                for (int tile = 0; tile < 3; ++tile) {
                     qf_engine::Launch C{};
                     C.cb = cb;
                     C.ck = ck;
@@ -1414,7 +1427,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
                     C.count = (int)tasks.size() - C.offset;
                     C.gcount = (int)groups.size() - C.goffset;
                     if (C.count == 0) continue;
-                    if (tile) {
+                    if (tile == 1) {
                         std::vector<long long> gc(C.gcount, 0);
                         for (int q = 0; q < C.gcount; ++q)
                             for (int x = groups[C.goffset + q].task_begin; x < groups[C.goffset + q].task_end; ++x)

@@ -1,0 +1,294 @@
+// Wave-per-quartet Fock kernel for the d-heavy classes.
+//
+// The per-lane kernels give every lane its own quartet, so a (dd|dd)
+// quartet's 961 accumulators and ~2,200 VRR intermediates live in scratch
+// memory and the launch is latency-bound at one wave per SIMD.  Here the 64
+// lanes of a wave share ONE quartet: the generated node programs (see
+// gen_eri.coop_tables) evaluate the VRR level by level - every node of a
+// level is independent, one node per lane - and then the HRR the same way,
+// with all intermediates in LDS.  Digestion is lane-parallel too: each lane
+// owns output elements of J_ab, J_cd, K_ac, K_ad, K_bc, K_bd and sums its
+// contraction over the integral block in LDS.
+//
+// The arithmetic is the same as in fock_task (same screening, same Boys
+// function, same primitive loops and prefactors, same digestion weights);
+// only the evaluation order of the recursions differs.
+#pragma once
+#include "qf_device.h"
+
+namespace qf {
+
+struct CoopVNode {
+    // value = C1 s[a] + C2 s[b] + n1 H (s[c] - Rr s[d]) + n2 (1/2(z+n)) s[e], written to s[dst];
+    // (C1, C2, H, Rr) = (PA, WP, 1/2z, rho/z) for sel = dir (bra step) and
+    // (QC, WQ, 1/2n, rho/n) for sel = 3 + dir (ket step)
+    unsigned short a, b, c, d, e, dst;
+    unsigned char sel, n1, n2, pad;
+};
+
+struct CoopHNode {
+    // s[dst] = s[s1] + X s[s2], X = CD[dir] (sel = dir) or AB[dir] (sel = 3 + dir)
+    unsigned short dst, s1, s2;
+    unsigned char sel, pad;
+};
+
+constexpr int kVU = 2;   // VRR nodes per thread in flight
+constexpr int kHU = 4;   // HRR nodes per thread in flight
+
+
+template <class D>
+__device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
+                                             int j, const PairInfo& ki, double dn, const MolDesc& md,
+                                             QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
+                                             const QF_AS_LDS double* boys_tab)
+{
+    constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
+    const int lane = threadIdx.x;   // thread index in the workgroup
+    QF_AS_LDS double* cf = s + D::CF;   // [0,24) VRR coefficients, 24: 1/2(z+n), [25,31): CD, AB
+    for (int x = lane; x < D::NACC; x += D::NT) s[x] = 0.0;
+    __syncthreads();
+
+    // primitive loops: QUICK's screen and order, as in contract()
+    const auto bp = cptr(bb.prim);
+    const auto kp = cptr(kb.prim);
+    const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
+    const int maxm = bi.qfs + ki.qfs;
+    const double c0 = dn * bp[(PF_CUT * bK) * bn + i];
+    const auto vrr = gptr(D::vrr());
+    const auto vlev = cptr(D::vlev());
+    const auto tgt = gptr(D::tgt());
+    for (int jp = 0; jp < kK; ++jp) {
+        const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
+        if (!(c0 * cutk > md.primlim)) break;
+        const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
+        const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
+        const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
+                              kp[(PF_PZ * kK + jp) * kn + j]};
+        const double Kk = kp[(PF_KC * kK + jp) * kn + j];
+        const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
+        for (int ip = 0; ip < bK; ++ip) {
+            if (!((dn * bp[(PF_CUT * bK + ip) * bn + i]) * cutk > md.primlim)) break;
+            const double zb = bp[(PF_ZETA * bK + ip) * bn + i];
+            const double ozb = bp[(PF_OOZ2 * bK + ip) * bn + i];
+            const double Pb[3] = {bp[(PF_PX * bK + ip) * bn + i], bp[(PF_PY * bK + ip) * bn + i],
+                                  bp[(PF_PZ * bK + ip) * bn + i]};
+            const double Kb = bp[(PF_KC * bK + ip) * bn + i];
+            const double rs = qf_rsqrt(zb + zk);
+            const double inv = rs * rs;
+            const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
+            const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+            const double T = rpq * (zb * zk * inv);
+            double F[D::L + 1];
+            boys_quick<D::L>(T, maxm, F, boys_tab);
+            const double pref = Kb * Kk * rs;
+            const double wp = zk * inv, wq = zb * inv;
+#pragma unroll
+            for (int m = 0; m <= D::L; ++m)
+                if (lane == m) s[D::BASE + m] = F[m] * pref;
+            if (lane < 24) {
+                const int sel = lane >> 2, k = lane & 3, dir = sel % 3;
+                const double pa = dir == 0 ? Pb[0] - bi.A[0] : dir == 1 ? Pb[1] - bi.A[1] : Pb[2] - bi.A[2];
+                const double pq = dir == 0 ? PQ[0] : dir == 1 ? PQ[1] : PQ[2];
+                const double qc = dir == 0 ? QC[0] : dir == 1 ? QC[1] : QC[2];
+                double v;
+                if (sel < 3)
+                    v = k == 0 ? pa : k == 1 ? -wp * pq : k == 2 ? ozb : wp;
+                else
+                    v = k == 0 ? qc : k == 1 ? wq * pq : k == 2 ? ozk : wq;
+                cf[lane] = v;
+            } else if (lane == 24) {
+                cf[24] = 0.5 * inv;
+            } else if (lane == 25) {
+                s[D::ZS] = 0.0;
+            }
+            __syncthreads();
+            for (int lv = 0; lv < D::NLV; ++lv) {
+                const int e1 = vlev[lv + 1];
+                // nodes of one level are independent: fetch kVU descriptors,
+                // read all their sources, then compute and store, so the
+                // memory latencies overlap instead of chaining node by node
+                for (int x0 = vlev[lv] + lane; x0 < e1; x0 += kVU * D::NT) {
+                    CoopVNode q[kVU];
+#pragma unroll
+                    for (int u = 0; u < kVU; ++u) {
+                        const int x = x0 + u * D::NT;
+                        q[u] = vrr[x < e1 ? x : e1 - 1];
+                    }
+                    double v[kVU];
+#pragma unroll
+                    for (int u = 0; u < kVU; ++u) {
+                        const QF_AS_LDS double* c4 = cf + 4 * q[u].sel;
+                        v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] +
+                               double(q[u].n1) * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
+                               double(q[u].n2) * cf[24] * s[q[u].e];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kVU; ++u)
+                        if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+                }
+                __syncthreads();
+            }
+            for (int x = lane; x < D::NACC; x += D::NT) s[x] += s[tgt[x]];
+            __syncthreads();
+        }
+    }
+
+    // HRR on the contracted block (the VRR region is dead now)
+    if (lane < 3) {
+        cf[25 + lane] = ki.AB[lane];
+        cf[28 + lane] = bi.AB[lane];
+    }
+    __syncthreads();
+    const auto hrr = gptr(D::hrr());
+    const auto hlev = cptr(D::hlev());
+    for (int lv = 0; lv < D::NLH; ++lv) {
+        const int e1 = hlev[lv + 1];
+        for (int x0 = hlev[lv] + lane; x0 < e1; x0 += kHU * D::NT) {
+            CoopHNode q[kHU];
+#pragma unroll
+            for (int u = 0; u < kHU; ++u) {
+                const int x = x0 + u * D::NT;
+                q[u] = hrr[x < e1 ? x : e1 - 1];
+            }
+            double v[kHU];
+#pragma unroll
+            for (int u = 0; u < kHU; ++u) v[u] = s[q[u].s1] + cf[25 + q[u].sel] * s[q[u].s2];
+#pragma unroll
+            for (int u = 0; u < kHU; ++u)
+                if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+        }
+        __syncthreads();
+    }
+
+    // integrals with cons and degeneracy weight; density blocks
+    const double sdeg = 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) * (bi.id != ki.id ? 2.0 : 1.0);
+    const auto out = gptr(D::out());
+    QF_AS_LDS double* Ic = s + D::IC;
+    for (int x = lane; x < D::NI; x += D::NT) {
+        const int d = x % nd, c = (x / nd) % nc, b = (x / (nd * nc)) % nb, a = x / (nd * nc * nb);
+        const double cons = kCons[D::La][a] * kCons[D::Lb][b] * kCons[D::Lc][c] * kCons[D::Ld][d];
+        Ic[x] = s[out[x]] * (cons * sdeg);
+    }
+    const int n = md.nbasis;
+    const auto P = gptr(md.P);
+    QF_AS_LDS double* Dab = s + D::DM;
+    QF_AS_LDS double* Dcd = Dab + na * nb;
+    QF_AS_LDS double* Dac = Dcd + nc * nd;
+    QF_AS_LDS double* Dad = Dac + na * nc;
+    QF_AS_LDS double* Dbc = Dad + na * nd;
+    QF_AS_LDS double* Dbd = Dbc + nb * nc;
+    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = P[(bi.fa + x / nb) * n + bi.fb + x % nb];
+    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = P[(ki.fa + x / nd) * n + ki.fb + x % nd];
+    for (int x = lane; x < na * nc; x += D::NT) Dac[x] = P[(bi.fa + x / nc) * n + ki.fa + x % nc];
+    for (int x = lane; x < na * nd; x += D::NT) Dad[x] = P[(bi.fa + x / nd) * n + ki.fb + x % nd];
+    for (int x = lane; x < nb * nc; x += D::NT) Dbc[x] = P[(bi.fb + x / nc) * n + ki.fa + x % nc];
+    for (int x = lane; x < nb * nd; x += D::NT) Dbd[x] = P[(bi.fb + x / nd) * n + ki.fb + x % nd];
+    __syncthreads();
+
+    // lane-owned contractions, one G element each
+    const double kscale = -0.25 * md.hyb;
+    constexpr int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
+                  o6 = o5 + nb * nd;
+#define IX(a, b, c, d) Ic[(((a) * nb + (b)) * nc + (c)) * nd + (d)]
+    for (int o = lane; o < o6; o += D::NT) {
+        double v = 0.0;
+        int row, col;
+        if (o < o1) {                              // J_ab = sum_cd D_cd I
+            const int a = o / nb, b = o % nb;
+            for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) v += Dcd[c * nd + d] * IX(a, b, c, d);
+            row = bi.fa + a;
+            col = bi.fb + b;
+        } else if (o < o2) {                       // J_cd = sum_ab D_ab I
+            const int c = (o - o1) / nd, d = (o - o1) % nd;
+            for (int a = 0; a < na; ++a)
+                for (int b = 0; b < nb; ++b) v += Dab[a * nb + b] * IX(a, b, c, d);
+            row = ki.fa + c;
+            col = ki.fb + d;
+        } else if (o < o3) {                       // K_ac = sum_bd D_bd I
+            const int a = (o - o2) / nc, c = (o - o2) % nc;
+            for (int b = 0; b < nb; ++b)
+                for (int d = 0; d < nd; ++d) v += Dbd[b * nd + d] * IX(a, b, c, d);
+            v *= kscale;
+            row = bi.fa + a;
+            col = ki.fa + c;
+        } else if (o < o4) {                       // K_ad = sum_bc D_bc I
+            const int a = (o - o3) / nd, d = (o - o3) % nd;
+            for (int b = 0; b < nb; ++b)
+                for (int c = 0; c < nc; ++c) v += Dbc[b * nc + c] * IX(a, b, c, d);
+            v *= kscale;
+            row = bi.fa + a;
+            col = ki.fb + d;
+        } else if (o < o5) {                       // K_bc = sum_ad D_ad I
+            const int b = (o - o4) / nc, c = (o - o4) % nc;
+            for (int a = 0; a < na; ++a)
+                for (int d = 0; d < nd; ++d) v += Dad[a * nd + d] * IX(a, b, c, d);
+            v *= kscale;
+            row = bi.fb + b;
+            col = ki.fa + c;
+        } else {                                   // K_bd = sum_ac D_ac I
+            const int b = (o - o5) / nd, d = (o - o5) % nd;
+            for (int a = 0; a < na; ++a)
+                for (int c = 0; c < nc; ++c) v += Dac[a * nc + c] * IX(a, b, c, d);
+            v *= kscale;
+            row = bi.fb + b;
+            col = ki.fb + d;
+        }
+        if (v != 0.0) atomic_add(&G[row * n + col], v);
+    }
+#undef IX
+    __syncthreads();   // the next quartet reuses the LDS
+}
+
+// One workgroup of D::NT threads per task (tasks of narrow ket windows):
+// 4 waves for the largest node programs, one wave otherwise.  Every wave
+// screens the task's quartets lane-parallel (the same tests, so the same
+// ballot in each wave), then the workgroup processes the surviving quartets
+// one after another.  FP64 throughout.  Dynamic LDS: D::LDS doubles.
+template <class D>
+__global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict__ tasks, int ntasks,
+                                                         const PairBlock* __restrict__ blocks,
+                                                         const MolDesc* __restrict__ mols)
+{
+    extern __shared__ double qf_lds[];
+    QF_BOYS_LDS_DECL;
+    QF_BOYS_LDS_FILL();
+    QF_AS_LDS double* s = (QF_AS_LDS double*)qf_lds;
+    const int lane = threadIdx.x % WAVE;
+    for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+        const Task t = cptr(tasks)[ti];
+        const PairBlock bb = cptr(blocks)[t.bra_block];
+        const PairBlock kb = cptr(blocks)[t.ket_block];
+        const MolDesc md = cptr(mols)[bb.mol];
+        const int ns = md.nshell;
+        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis);
+        const auto cm = gptr(md.cutmat);
+        const auto bperm = cptr(bb.bperm);
+        const bool in_range = lane < t.ket_count;
+        const int jl = t.ket_start + (in_range ? lane : 0);
+        const PairInfo kl = gptr(kb.info)[jl];
+        for (int kk = t.kb; kk < t.ke; ++kk) {
+            const int i = bperm[kk];
+            const PairInfo bi = cptr(bb.info)[i];
+            const double ytest = bi.ycut * kl.ycut;
+            double dn = fmax(4.0 * cm[bi.sa * ns + bi.sb], 4.0 * cm[kl.sa * ns + kl.sb]);
+            dn = fmax(dn, cm[bi.sa * ns + kl.sb]);
+            dn = fmax(dn, cm[bi.sa * ns + kl.sa]);
+            dn = fmax(dn, cm[bi.sb * ns + kl.sa]);
+            dn = fmax(dn, cm[bi.sb * ns + kl.sb]);
+            const bool valid = in_range && (ytest > md.cutoff) && (ytest * dn > md.cutoff) &&
+                               (!t.same_block || jl <= i) && (t.all_fp64 || ytest >= md.split);
+            unsigned long long mask = __ballot(valid);
+            while (mask) {
+                const int v = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int j = t.ket_start + v;
+                const PairInfo ki = cptr(kb.info)[j];
+                const double dnv = __shfl(dn, v, WAVE);
+                coop_quartet<D>(bb, i, bi, kb, j, ki, dnv, md, G, s, QF_BOYS_LDS_PTR);
+            }
+        }
+    }
+}
+
+}  // namespace qf

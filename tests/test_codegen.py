@@ -1,0 +1,106 @@
+"""The node programs of the wave-per-quartet kernels (level-scheduled VRR
+and HRR over LDS slots) compute exactly the recursions of the class: run
+them with a small interpreter on random recursion parameters and compare
+with a direct evaluation of the same Head-Gordon-Pople recurrences."""
+import numpy as np
+import pytest
+
+from quick_amd.codegen import gen_eri as G
+
+
+def reference(La, Lb, Lc, Ld, F, p):
+    """Direct recursive evaluation of the VRR (HGP eqs. 6, 16) and HRR."""
+    memo = {}
+
+    def vrr(e, f, m):
+        k = (e, f, m)
+        if k in memo:
+            return memo[k]
+        if e == G.Z and f == G.Z:
+            r = F[m]
+        elif f == G.Z:
+            i = next(j for j in range(3) if e[j] > 0)
+            i = min((j for j in range(3) if e[j] > 0), key=lambda j: (e[j] > 1, -e[j]))
+            e1 = G.sub(e, i)
+            r = p["PA"][i] * vrr(e1, f, m) + p["WP"][i] * vrr(e1, f, m + 1)
+            if e1[i] > 0:
+                e2 = G.sub(e1, i)
+                r += e1[i] * p["oo2z"] * (vrr(e2, f, m) - p["rz"] * vrr(e2, f, m + 1))
+        else:
+            i = min((j for j in range(3) if f[j] > 0), key=lambda j: (e[j] > 0, f[j] > 1, -f[j]))
+            f1 = G.sub(f, i)
+            r = p["QC"][i] * vrr(e, f1, m) + p["WQ"][i] * vrr(e, f1, m + 1)
+            if f1[i] > 0:
+                f2 = G.sub(f1, i)
+                r += f1[i] * p["oo2n"] * (vrr(e, f2, m) - p["rn"] * vrr(e, f2, m + 1))
+            if e[i] > 0:
+                r += e[i] * p["oo2zn"] * vrr(G.sub(e, i), f1, m + 1)
+        memo[k] = r
+        return r
+
+    def ket(e, c, d):
+        if d == G.Z:
+            return vrr(e, c, 0)
+        i = next(j for j in range(3) if d[j] > 0)
+        return ket(e, G.add(c, i), G.sub(d, i)) + p["CD"][i] * ket(e, c, G.sub(d, i))
+
+    def bra(a, b, c, d):
+        if b == G.Z:
+            return ket(a, c, d)
+        i = next(j for j in range(3) if b[j] > 0)
+        return bra(G.add(a, i), G.sub(b, i), c, d) + p["AB"][i] * bra(a, G.sub(b, i), c, d)
+
+    return np.array([bra(*o) for o in G.coop_tables(La, Lb, Lc, Ld)["outs"]])
+
+
+def interpret(t, F, p):
+    """Execute the node programs the way fock_coop_kernel does."""
+    s = np.full(t["total"] + 8, np.nan)
+    nacc = t["nacc"]
+    s[:nacc] = 0.0
+    s[t["zs"]] = 0.0
+    for m in range(t["L"] + 1):
+        s[nacc + m] = F[m]
+    cf = np.zeros(6 * 4)
+    for sel in range(6):
+        op, d = sel // 3, sel % 3
+        cf[sel * 4:sel * 4 + 4] = ([p["PA"][d], p["WP"][d], p["oo2z"], p["rz"]] if op == 0 else
+                                   [p["QC"][d], p["WQ"][d], p["oo2n"], p["rn"]])
+    lev = t["vlev"]
+    for lv in range(len(lev) - 1):
+        written = []
+        for (a, b, c, d, e, dst, sel, n1, n2) in t["vnodes"][lev[lv]:lev[lv + 1]]:
+            c4 = cf[sel * 4:sel * 4 + 4]
+            for src in (a, b, c, d, e):
+                assert src not in written, "node reads a slot written in the same level"
+            val = c4[0] * s[a] + c4[1] * s[b] + n1 * c4[2] * (s[c] - c4[3] * s[d]) + n2 * p["oo2zn"] * s[e]
+            written.append(dst)
+            s[dst] = val
+    acc = s[t["tgt"]].copy()
+    s[:nacc] = acc
+    s[nacc:t["cf"]] = np.nan          # the HRR overwrites the VRR region
+    hcf = list(p["CD"]) + list(p["AB"])
+    lev = t["hlev"]
+    for lv in range(len(lev) - 1):
+        written = []
+        for (dst, s1, s2, sel, _) in [x + (0,) for x in t["hnodes"][lev[lv]:lev[lv + 1]]]:
+            assert s1 not in written and s2 not in written
+            s[dst] = s[s1] + hcf[sel] * s[s2]
+            written.append(dst)
+    return s[t["omap"]]
+
+
+@pytest.mark.parametrize("cls", G.COOP_CLASSES, ids=lambda c: "%d%d%d%d" % c)
+def test_coop_node_programs_match_recursion(cls):
+    rng = np.random.default_rng(sum(cls) * 7 + 1)
+    t = G.coop_tables(*cls)
+    L = t["L"]
+    F = rng.uniform(0.1, 1.0, L + 1)
+    p = {k: rng.uniform(-1, 1, 3) for k in ("PA", "WP", "QC", "WQ", "AB", "CD")}
+    for k in ("oo2z", "oo2n", "oo2zn", "rz", "rn"):
+        p[k] = float(rng.uniform(0.1, 0.9))
+    ref = reference(*cls, F, p)
+    got = interpret(t, F, p)
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+    assert t["total"] * 8 <= 64 * 1024

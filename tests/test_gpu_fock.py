@@ -61,13 +61,15 @@ def test_fock_matches_oracle(name, basis, cutoff):
     assert np.max(np.abs(G - G.T)) <= 1e-14 * max(1.0, np.max(np.abs(G)))
 
 
-@pytest.mark.parametrize("mode", ["direct", "tile-small", "tile-1wave"])
-def test_digestion_paths_match_oracle(mode):
+@pytest.mark.parametrize("mode,name", [("direct", "water3"), ("tile-small", "water3"), ("tile-1wave", "water3"),
+                                       ("coop", "water3"), ("coop", "gly1"), ("coop-4bras", "alkane2")])
+def test_digestion_paths_match_oracle(mode, name):
     """The direct-atomic kernels, LDS tiles forced small (some classes fall
-    back, chunks shrink) and one-wave tile workgroups all reproduce the
-    oracle on a cluster with s, p and d classes."""
-    announce(f"digestion path {mode} water3 6-31G*")
-    mol = molecule("water3")
+    back, chunks shrink), one-wave tile workgroups and the wave-per-quartet
+    kernels (d-heavy classes; the rest on tiles) all reproduce the oracle on
+    molecules with s, p and d classes."""
+    announce(f"digestion path {mode} {name} 6-31G*")
+    mol = molecule(name)
     orc = oracle_for(mol, "6-31G*")
     P, _ = seeded_density_for(orc, mol)
     G_ref, _ = orc.fock2e(P, cutoff=1e-12, threads=4)
@@ -76,8 +78,12 @@ def test_digestion_paths_match_oracle(mode):
         eng.set_tile(False)
     elif mode == "tile-small":
         eng.set_tile(True, waves=2, tasks_per_group=3, lds_bytes=4096)
-    else:
+    elif mode == "tile-1wave":
         eng.set_tile(True, waves=1, tasks_per_group=1)
+    else:
+        for cb in range(6):
+            for ck in range(cb + 1):
+                eng.set_class_tuning(cb, ck, 2, 0, 0, 4 if mode == "coop-4bras" else 0)
     G = eng.fock2e(P)
     err = float(np.max(np.abs(G - G_ref)))
     print(f"  max|dF| = {err:.3e}", flush=True)
