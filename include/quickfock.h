@@ -144,6 +144,12 @@ int qf_batch_read_timing(qf_batch *b, double *ms_per_launch, int *calls_per_laun
  * with nuclear charges[natom]; computed on the GPU. */
 int qf_one_electron(qf_engine *e, const double *charges, double *S, double *T, double *V);
 
+/* Same with an explicit list of point charges (the atoms and any external
+ * charges, QUICK's allxyz/allchg): V = -sum_C Z_C <mu|1/|r-C||nu>.
+ * center_xyz[3*ncenter] in bohr. */
+int qf_one_electron_ext(qf_engine *e, int ncenter, const double *center_xyz, const double *charges, double *S,
+                        double *T, double *V);
+
 /* Algorithmic FLOPs of a build with the density of the last build: the
  * quartets and primitive quartets that pass the screens, counted on the
  * device, priced with the generated per-class FLOP table. */
@@ -220,6 +226,18 @@ void gpu_upload_basis_(int *nshell, int *nprim, int *jshell, int *jbasis, int *m
                        double *gccoeff, double *cons, double *gcexpo, int *KLMN);
 /* gpu.cu:879 gpu_upload_cutoff_matrix_ (YCutoff(nshell,nshell), cutPrim(jbasis,jbasis)) */
 void gpu_upload_cutoff_matrix_(double *YCutoff, double *cutPrim);
+/* gpu.cu:1763 gpu_upload_density_matrix_ (the density of the next build) */
+void gpu_upload_density_matrix_(double *dense);
+/* gpu.cu:1490 gpu_upload_oei_ (external point charges: extxyz(3, nextatom), extchg(nextatom)) */
+void gpu_upload_oei_(int *nextatom, double *extxyz, double *extchg, int *ierr);
+/* gpu_get2e_getxc_drivers.h:360 gpu_get_oei_: o += V, the nuclear attraction of the atoms and the
+ * external charges (the kinetic part stays on the CPU, quick_oei_module.f90:95-125) */
+void gpu_get_oei_(double *o);
+/* gpu.cu:405 gpu_get_device_info_, with the argument list hip/gpu_write_info.f90:33-34 passes
+ * (device name and architecture name are Fortran character(len=256)) */
+void gpu_get_device_info_(int *gpu_dev_count, int *gpu_dev_id, int *gpu_dev_mem, int *gpu_num_proc,
+                          double *gpu_core_freq, char *gpu_dev_name, int *name_len, char *gpu_arch_name,
+                          int *arch_len, int *majorv, int *minorv, int *ierr);
 /* gpu.cu:1690 gpu_upload_calculated_ (only dense is read) */
 void gpu_upload_calculated_(double *o, double *co, double *vec, double *dense);
 /* gpu.cu:840 gpu_upload_cutoff_ */

@@ -150,7 +150,8 @@ extern "C" int qf_internal_basis(qf_engine* e, int* natom, const double** xyz, i
                                  const int** s_fstart, int* nprim, const double** exps, const double** coef,
                                  int* nbasis);
 
-extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, double* T, double* V)
+extern "C" int qf_one_electron_ext(qf_engine* e, int ncenter, const double* center_xyz, const double* charges,
+                                   double* S, double* T, double* V)
 {
     int natom, nshell, nprim, nbasis;
     const double *xyz, *exps, *coef;
@@ -158,6 +159,7 @@ extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, d
     int rc = qf_internal_basis(e, &natom, &xyz, &nshell, &s_atom, &s_lmin, &s_lmax, &s_nprim, &s_pstart, &s_fstart,
                                &nprim, &exps, &coef, &nbasis);
     if (rc) return rc;
+    if (ncenter < 0 || (ncenter > 0 && (!center_xyz || !charges)) || !S || !T || !V) return QF_ERR_ARG;
     static const int comp[3][6][3] = {{{0, 0, 0}},
                                       {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
                                       {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}}};
@@ -174,27 +176,30 @@ extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, d
             }
     }
     const size_t n2 = (size_t)nbasis * nbasis;
-    OeiFn* dfn;
-    double *dex, *dco, *dxyz, *dch, *dS, *dT, *dV;
-    if (hipMalloc(&dfn, sizeof(OeiFn) * nbasis) != hipSuccess) return QF_ERR_HIP;
-    (void)hipMalloc(&dex, sizeof(double) * nprim);
-    (void)hipMalloc(&dco, sizeof(double) * 4 * nprim);
-    (void)hipMalloc(&dxyz, sizeof(double) * 3 * natom);
-    (void)hipMalloc(&dch, sizeof(double) * natom);
-    (void)hipMalloc(&dS, sizeof(double) * n2 * 3);
-    dT = dS + n2;
-    dV = dT + n2;
-    (void)hipMemcpy(dfn, fn.data(), sizeof(OeiFn) * nbasis, hipMemcpyHostToDevice);
-    (void)hipMemcpy(dex, exps, sizeof(double) * nprim, hipMemcpyHostToDevice);
-    (void)hipMemcpy(dco, coef, sizeof(double) * 4 * nprim, hipMemcpyHostToDevice);
-    (void)hipMemcpy(dxyz, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice);
-    (void)hipMemcpy(dch, charges, sizeof(double) * natom, hipMemcpyHostToDevice);
-    hipLaunchKernelGGL(k_oei, dim3((n2 + 127) / 128), dim3(128), 0, 0, dfn, nbasis, dex, dco, natom, dxyz, dch, dS,
-                       dT, dV);
-    hipError_t err = hipGetLastError();
-    if (err == hipSuccess) err = hipMemcpy(S, dS, sizeof(double) * n2, hipMemcpyDeviceToHost);
-    if (err == hipSuccess) err = hipMemcpy(T, dT, sizeof(double) * n2, hipMemcpyDeviceToHost);
-    if (err == hipSuccess) err = hipMemcpy(V, dV, sizeof(double) * n2, hipMemcpyDeviceToHost);
+    const int nc = std::max(ncenter, 1);
+    OeiFn* dfn = nullptr;
+    double *dex = nullptr, *dco = nullptr, *dxyz = nullptr, *dch = nullptr, *dS = nullptr;
+    hipError_t err = hipMalloc(&dfn, sizeof(OeiFn) * nbasis);
+    if (err == hipSuccess) err = hipMalloc(&dex, sizeof(double) * nprim);
+    if (err == hipSuccess) err = hipMalloc(&dco, sizeof(double) * 4 * nprim);
+    if (err == hipSuccess) err = hipMalloc(&dxyz, sizeof(double) * 3 * nc);
+    if (err == hipSuccess) err = hipMalloc(&dch, sizeof(double) * nc);
+    if (err == hipSuccess) err = hipMalloc(&dS, sizeof(double) * n2 * 3);
+    if (err == hipSuccess) err = hipMemcpy(dfn, fn.data(), sizeof(OeiFn) * nbasis, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(dex, exps, sizeof(double) * nprim, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(dco, coef, sizeof(double) * 4 * nprim, hipMemcpyHostToDevice);
+    if (err == hipSuccess && ncenter > 0) err = hipMemcpy(dxyz, center_xyz, sizeof(double) * 3 * ncenter, hipMemcpyHostToDevice);
+    if (err == hipSuccess && ncenter > 0) err = hipMemcpy(dch, charges, sizeof(double) * ncenter, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        double* dT = dS + n2;
+        double* dV = dT + n2;
+        hipLaunchKernelGGL(k_oei, dim3((n2 + 127) / 128), dim3(128), 0, 0, dfn, nbasis, dex, dco, ncenter, dxyz, dch,
+                           dS, dT, dV);
+        err = hipGetLastError();
+        if (err == hipSuccess) err = hipMemcpy(S, dS, sizeof(double) * n2, hipMemcpyDeviceToHost);
+        if (err == hipSuccess) err = hipMemcpy(T, dT, sizeof(double) * n2, hipMemcpyDeviceToHost);
+        if (err == hipSuccess) err = hipMemcpy(V, dV, sizeof(double) * n2, hipMemcpyDeviceToHost);
+    }
     (void)hipFree(dfn);
     (void)hipFree(dex);
     (void)hipFree(dco);
@@ -202,4 +207,15 @@ extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, d
     (void)hipFree(dch);
     (void)hipFree(dS);
     return err == hipSuccess ? QF_OK : QF_ERR_HIP;
+}
+
+extern "C" int qf_one_electron(qf_engine* e, const double* charges, double* S, double* T, double* V)
+{
+    int natom, nshell, nprim, nbasis;
+    const double *xyz, *exps, *coef;
+    const int *s_atom, *s_lmin, *s_lmax, *s_nprim, *s_pstart, *s_fstart;
+    int rc = qf_internal_basis(e, &natom, &xyz, &nshell, &s_atom, &s_lmin, &s_lmax, &s_nprim, &s_pstart, &s_fstart,
+                               &nprim, &exps, &coef, &nbasis);
+    if (rc) return rc;
+    return qf_one_electron_ext(e, natom, xyz, charges, S, T, V);
 }

@@ -6,6 +6,7 @@
 // (src/gpu/hip/gpu_type.h:511-512).  Not re-entrant, one molecule at a time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,6 +30,8 @@ struct QuickState {
     bool oshell = false;
     int nshell = 0, jbasis = 0;
     std::vector<double> dense, cutmat;
+    std::vector<double> xyz, chg;          // atoms (bohr) and nuclear charges
+    std::vector<double> ext_xyz, ext_chg;  // external point charges (gpu_upload_oei_)
 };
 
 QuickState* g_state = nullptr;
@@ -93,10 +96,87 @@ void gpu_setup_(int* natom, int* nbasis, int* /*nElec*/, int* /*imult*/, int* /*
 void gpu_upload_xyz_(double* atom_xyz)
 {
     if (!g_state || !g_state->eng) return;
+    g_state->xyz.assign(atom_xyz, atom_xyz + 3 * (size_t)g_state->natom);
     if (qf_set_molecule(g_state->eng, g_state->natom, atom_xyz)) report("gpu_upload_xyz_");
 }
 
-void gpu_upload_atom_and_chg_(int*, double*) {}
+void gpu_upload_atom_and_chg_(int* /*atom*/, double* atom_chg)
+{
+    if (!g_state) return;
+    g_state->chg.assign(atom_chg, atom_chg + g_state->natom);
+}
+
+void gpu_upload_oei_(int* nextatom, double* extxyz, double* extchg, int* ierr)
+{
+    if (ierr) *ierr = 0;
+    if (!g_state) {
+        if (ierr) *ierr = 1;
+        return;
+    }
+    const int m = nextatom ? *nextatom : 0;
+    g_state->ext_xyz.assign(extxyz, extxyz + 3 * (size_t)std::max(m, 0));
+    g_state->ext_chg.assign(extchg, extchg + std::max(m, 0));
+}
+
+void gpu_get_oei_(double* o)
+{
+    // QUICK adds the nuclear attraction of atoms and external charges into o
+    // (gpu_get2e_getxc_drivers.h:360-392); the kinetic part is the CPU's.
+    if (!g_state || !g_state->eng || g_state->chg.size() != (size_t)g_state->natom) {
+        std::fprintf(stderr, "quickfock: gpu_get_oei_ called before gpu_upload_xyz_/gpu_upload_atom_and_chg_\n");
+        std::exit(EXIT_FAILURE);
+    }
+    std::vector<double> cx(g_state->xyz), cq(g_state->chg);
+    cx.insert(cx.end(), g_state->ext_xyz.begin(), g_state->ext_xyz.end());
+    cq.insert(cq.end(), g_state->ext_chg.begin(), g_state->ext_chg.end());
+    const size_t n2 = (size_t)g_state->nbasis * g_state->nbasis;
+    std::vector<double> S(n2), T(n2), V(n2);
+    if (qf_one_electron_ext(g_state->eng, (int)cq.size(), cx.data(), cq.data(), S.data(), T.data(), V.data()))
+        fatal("gpu_get_oei_");
+    for (size_t k = 0; k < n2; ++k) o[k] += V[k];
+}
+
+void gpu_upload_density_matrix_(double* dense)
+{
+    if (!g_state) return;
+    g_state->dense.assign(dense, dense + (size_t)g_state->nbasis * g_state->nbasis);
+}
+
+void gpu_get_device_info_(int* gpu_dev_count, int* gpu_dev_id, int* gpu_dev_mem, int* gpu_num_proc,
+                          double* gpu_core_freq, char* gpu_dev_name, int* name_len, char* gpu_arch_name,
+                          int* arch_len, int* majorv, int* minorv, int* ierr)
+{
+    if (ierr) *ierr = 0;
+    *gpu_dev_count = 0;
+    if (hipGetDeviceCount(gpu_dev_count) != hipSuccess || *gpu_dev_count == 0) {
+        *gpu_dev_count = 0;
+        if (ierr) *ierr = 24;
+        return;
+    }
+    const int dev = g_state ? g_state->dev : 0;
+    *gpu_dev_id = dev;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        if (ierr) *ierr = 25;
+        return;
+    }
+    *gpu_dev_mem = (int)(prop.totalGlobalMem / (1024 * 1024));
+    *gpu_num_proc = prop.multiProcessorCount;
+    int khz = 0;
+    (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, dev);
+    *gpu_core_freq = khz * 1e-6;
+    // Fortran character(len=256) buffers: blank padded, no terminator needed
+    const int cap = 256;
+    const int nl = (int)strnlen(prop.name, cap), al = (int)strnlen(prop.gcnArchName, cap);
+    std::memset(gpu_dev_name, ' ', cap);
+    std::memcpy(gpu_dev_name, prop.name, nl);
+    *name_len = nl;
+    std::memset(gpu_arch_name, ' ', cap);
+    std::memcpy(gpu_arch_name, prop.gcnArchName, al);
+    *arch_len = al;
+    *majorv = prop.major;
+    *minorv = prop.minor;
+}
 
 void gpu_upload_basis_(int* nshell, int* /*nprim*/, int* /*jshell*/, int* jbasis, int* /*maxcontract*/,
                        int* /*ncontract*/, int* /*itype*/, double* /*aexp*/, double* /*dcoeff*/,

@@ -249,3 +249,82 @@ def test_quick_fortran_abi_drop_in():
     lib.gpu_get_cshell_eri_(ctypes.byref(f), o.ctypes.data_as(ctypes.c_void_p))
     lib.gpu_delete_(ctypes.byref(ierr))
     assert np.max(np.abs((o - 0.25) - G_ref)) <= FOCK_TOL
+
+
+def test_quick_oei_and_device_info_entry_points():
+    """gpu_get_oei_ adds the nuclear attraction of the atoms and of external
+    point charges into o (QUICK's gpu_get2e_getxc_drivers.h:360); an
+    external charge q on top of atom C must equal raising Z_C by q.
+    gpu_get_device_info_ fills QUICK's fields."""
+    announce("QUICK OEI / device-info entry points, water 6-31G*")
+    from quick_amd import _lib
+    lib = _lib.lib()
+    mol = molecule("water")
+    orc = oracle_for(mol, "6-31G*")
+    S_ref, T_ref, V_ref = orc.one_electron()
+    qb = orc.qb
+    n = qb.nbasis
+    I = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+    ierr = ctypes.c_int(0)
+    lib.gpu_new_(ctypes.byref(ierr))
+    lib.gpu_set_device_(I(0), ctypes.byref(ierr))
+    lib.gpu_init_device_(ctypes.byref(ierr))
+    assert ierr.value == 0
+    # device info (Fortran character(len=256) buffers)
+    cnt, dev, mem, nproc, nl, al, mj, mn = (ctypes.c_int() for _ in range(8))
+    freq = ctypes.c_double()
+    name, arch = ctypes.create_string_buffer(256), ctypes.create_string_buffer(256)
+    lib.gpu_get_device_info_(ctypes.byref(cnt), ctypes.byref(dev), ctypes.byref(mem), ctypes.byref(nproc),
+                             ctypes.byref(freq), name, ctypes.byref(nl), arch, ctypes.byref(al), ctypes.byref(mj),
+                             ctypes.byref(mn), ctypes.byref(ierr))
+    assert ierr.value == 0 and cnt.value >= 1 and nproc.value > 0 and mem.value > 0
+    assert "gfx950" in arch.raw[:al.value].decode()
+    print(f"  {name.raw[:nl.value].decode()} {arch.raw[:al.value].decode()} CUs {nproc.value}", flush=True)
+    # basis upload through the QUICK shim, then o += V
+    lib.gpu_setup_(I(len(mol.symbols)), I(n), I(10), I(1), I(0), I(2))
+    xyz = np.ascontiguousarray(qb.xyz)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.gpu_upload_xyz_(vp(xyz))
+    chg = np.ascontiguousarray(qb.charge, dtype=np.float64)
+    atoms = np.zeros(len(mol.symbols), dtype=np.int32)
+    lib.gpu_upload_atom_and_chg_(vp(atoms), vp(chg))
+    MAXPRIM = 14
+    gccoeff = np.zeros((n, MAXPRIM))
+    gcexpo = np.zeros((n, MAXPRIM))
+    for s in range(qb.nshell):
+        f0 = qb.ksumtype[s]
+        nf = sum((L + 1) * (L + 2) // 2 for L in range(qb.qstart[s], qb.qfinal[s] + 1))
+        p0, k = qb.kstart[s], qb.kprim[s]
+        for fi in range(f0, f0 + nf):
+            gcexpo[fi, :k] = qb.gcexpo[p0:p0 + k]
+        for L in range(qb.qstart[s], qb.qfinal[s] + 1):
+            gccoeff[f0 + L, :k] = qb.gccoeff[p0:p0 + k, L]
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    dummy_i = np.zeros(max(n, qb.nshell) * 4 + 8, dtype=np.int32)
+    dummy_d = np.zeros(max(n, 1) * 4, dtype=np.float64)
+    lib.gpu_upload_basis_.restype = None
+    lib.gpu_upload_basis_(I(qb.nshell), I(qb.jbasis), I(qb.nshell), I(qb.jbasis), I(int(qb.kprim.max())),
+                          vp(dummy_i), vp(dummy_i), vp(dummy_d), vp(dummy_d), vp(dummy_i), vp(dummy_i), vp(dummy_i),
+                          vp(dummy_i), vp(dummy_i), vp(i32(qb.kstart + 1)), vp(i32(qb.katom + 1)), vp(dummy_i),
+                          vp(i32(qb.kprim)), vp(dummy_i), vp(i32(qb.ksumtype + 1)), vp(dummy_i), vp(i32(qb.qstart)),
+                          vp(i32(qb.qfinal)), vp(dummy_i), vp(dummy_i), vp(gccoeff), vp(dummy_d), vp(gcexpo),
+                          vp(dummy_i))
+    o = np.asfortranarray(np.full((n, n), 0.5))
+    lib.gpu_upload_oei_(I(0), vp(dummy_d), vp(dummy_d), ctypes.byref(ierr))
+    lib.gpu_get_oei_(vp(o))
+    assert np.max(np.abs((o - 0.5) - V_ref)) <= 1e-11
+    # an external charge of 0.75 on the oxygen == Z_O + 0.75
+    ext = np.ascontiguousarray(qb.xyz[0].copy())
+    q = np.array([0.75])
+    lib.gpu_upload_oei_(I(1), vp(ext), vp(q), ctypes.byref(ierr))
+    o2 = np.asfortranarray(np.zeros((n, n)))
+    lib.gpu_get_oei_(vp(o2))
+    chg2 = chg.copy()
+    chg2[0] += 0.75
+    lib.gpu_upload_oei_(I(0), vp(dummy_d), vp(dummy_d), ctypes.byref(ierr))
+    lib.gpu_upload_atom_and_chg_(vp(atoms), vp(chg2))
+    o3 = np.asfortranarray(np.zeros((n, n)))
+    lib.gpu_get_oei_(vp(o3))
+    assert np.max(np.abs(o2 - o3)) <= 1e-12
+    lib.gpu_cleanup_()
+    lib.gpu_delete_(ctypes.byref(ierr))
