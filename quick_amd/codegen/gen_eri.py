@@ -405,10 +405,11 @@ def quartet_classes():
 
 
 def min_waves(nacc, ni):
-    """Occupancy target per class (caps VGPRs: 8 waves -> 64, 4 -> 128, 2 -> 256)."""
+    """Occupancy target per class (caps VGPRs: 4 waves -> 128, 3 -> 168, 2 -> 256).
+    Even (ss|ss) needs ~150 VGPRs for the Boys function and the digestion
+    state, so no class is capped below 168 (at 128 the spills cost more than
+    the fourth wave gains)."""
     size = nacc + ni
-    if size <= 8:
-        return 4
     if size <= 40:
         return 3
     if size <= 80:

@@ -171,9 +171,9 @@ __device__ __forceinline__ void lds_add(QF_AS_LDS double* p, double v)
 // primitive ends the outer loop: the same set QUICK skips.
 //
 // The ket primitive (different per lane) is the outer loop: its seven
-// fields are vector-loaded once and the next one is prefetched while the
-// current one is used; the bra primitives (the same for the whole wave) are
-// the inner loop and come through scalar loads.
+// fields are vector-loaded once per ket primitive (prefetching the next one
+// cost 23 VGPRs and was slower); the bra primitives (the same for the whole
+// wave) are the inner loop and come through scalar loads.
 // ---------------------------------------------------------------------------
 template <class C, typename R, bool BRA_UNIFORM, typename Tab>
 __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
@@ -186,23 +186,14 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
     const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
 #define BP(idx) (BRA_UNIFORM ? bpc[idx] : bp[idx])
     const double c0 = dn * BP((PF_CUT * bK) * bn + i);     // largest bra cutprim
-    double nz = kp[(PF_ZETA * kK) * kn + j], noz = kp[(PF_OOZ2 * kK) * kn + j];
-    double nqx = kp[(PF_PX * kK) * kn + j], nqy = kp[(PF_PY * kK) * kn + j], nqz = kp[(PF_PZ * kK) * kn + j];
-    double nk = kp[(PF_KC * kK) * kn + j], nc = kp[(PF_CUT * kK) * kn + j];
     for (int jp = 0; jp < kK; ++jp) {
-        const double zk = nz, ozk = noz, Kk = nk, cutk = nc;
-        const double Qk[3] = {nqx, nqy, nqz};
-        if (jp + 1 < kK) {
-            const int o = jp + 1;
-            nz = kp[(PF_ZETA * kK + o) * kn + j];
-            noz = kp[(PF_OOZ2 * kK + o) * kn + j];
-            nqx = kp[(PF_PX * kK + o) * kn + j];
-            nqy = kp[(PF_PY * kK + o) * kn + j];
-            nqz = kp[(PF_PZ * kK + o) * kn + j];
-            nk = kp[(PF_KC * kK + o) * kn + j];
-            nc = kp[(PF_CUT * kK + o) * kn + j];
-        }
+        const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
         if (screen && !(c0 * cutk > primlim)) break;
+        const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
+        const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
+        const double Kk = kp[(PF_KC * kK + jp) * kn + j];
+        const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
+                              kp[(PF_PZ * kK + jp) * kn + j]};
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
         for (int ip = 0; ip < bK; ++ip) {
             if (screen && !((dn * BP((PF_CUT * bK + ip) * bn + i)) * cutk > primlim)) break;
