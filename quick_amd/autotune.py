@@ -15,15 +15,10 @@ import numpy as np
 # atomics, 1 LDS tile, 2 wave per quartet (classes without that kernel use
 # the tile); 0 elsewhere = engine default
 CANDIDATES = [
-    (2, 0, 0, 0),
-    (2, 0, 0, 4),
-    (0, 0, 0, 0),
-    (1, 4, 16, 0),
-    (1, 4, 8, 0),
-    (1, 2, 8, 0),
-    (1, 1, 4, 0),
-    (0, 0, 0, 4),
-    (1, 4, 32, 0),
+    (0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 0, 2), (0, 0, 0, 4),
+    (1, 4, 16, 0), (1, 4, 8, 0), (1, 2, 8, 0), (1, 1, 4, 0), (1, 4, 32, 0),
+    (1, 4, 16, 2), (1, 4, 16, 4), (1, 4, 32, 2), (1, 4, 32, 4),
+    (2, 0, 0, 0), (2, 0, 0, 1), (2, 0, 0, 2),
 ]
 
 
@@ -84,4 +79,5 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
         for e in engs:
             e.set_class_tuning(cb, ck, *cand)
         chosen[name] = (cand, total(cand))
+    autotune.last_times = times          # {candidate: {(class, fp32): ms}} of the last call
     return chosen
