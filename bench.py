@@ -68,6 +68,7 @@ def main():
     from quick_amd import molecules as M
     from quick_amd.engine import FockEngine
     from quick_amd.scf import rhf
+    from quick_amd.scf import seeded_density as seeded_density_np
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -100,17 +101,37 @@ def main():
         f"fp32 {st['n_tasks_fp32']}")
 
     # ---- SCF (reported, not part of the throughput number) ---------------
+    # plain: full FP64 builds every iteration, DIIS and diagonalisation on the
+    # host; device: every matrix on the GPU, incremental builds (QUICK's
+    # deltaO) with a full rebuild every 8 iterations and at convergence, in
+    # FP64 and in mixed precision
     scf = None
     S, T, V = eng.one_electron()
     if not a.no_scf:
+        from quick_amd.scf import rhf_device
+        eng.fock2e(seeded_density_np(S, mol.nelec() // 2))          # load the kernels once
         res = rhf(eng, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
+        log(f"[bench] SCF (plain) E = {res.energy:.10f} in {res.iterations} it, {res.wall:.2f}s")
+        P_host = res.density
         scf = {"energy_eh": res.energy, "iterations": res.iterations, "converged": bool(res.converged),
                "wall_s": round(res.wall, 3), "fock_ms_mean": round(1e3 * float(np.mean(res.fock_times)), 3)}
-        log(f"[bench] SCF E = {res.energy:.10f} in {res.iterations} it, {res.wall:.2f}s")
-        P_host = res.density
+        Sd = torch.tensor(S, device=dev)
+        torch.linalg.eigh(Sd)                                         # warm the device eigensolver
+        torch.cuda.synchronize()
+        for prec in ("fp64", "mixed"):
+            e2 = FockEngine(bs, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, precision=prec)
+            e2.fock2e(P_host)
+            r2 = rhf_device(e2, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
+            log(f"[bench] SCF (device, incremental, {prec}) E = {r2.energy:.10f} in {r2.iterations} it "
+                f"({r2.full_builds} full), {r2.wall:.2f}s")
+            scf[f"device_{prec}"] = {"energy_eh": r2.energy, "dE_vs_plain_eh": abs(r2.energy - res.energy),
+                                     "iterations": r2.iterations, "full_builds": r2.full_builds,
+                                     "converged": bool(r2.converged), "wall_s": round(r2.wall, 3),
+                                     "fock_diis_ms_mean": round(1e3 * float(np.mean(r2.fock_times)), 3),
+                                     "speedup_vs_plain": round(res.wall / r2.wall, 2)}
+            e2.close()
     else:
-        from quick_amd.scf import seeded_density
-        P_host = seeded_density(S, mol.nelec() // 2)
+        P_host = seeded_density_np(S, mol.nelec() // 2)
 
     # ---- timed Fock builds -------------------------------------------------
     n = bs.nbasis

@@ -60,7 +60,7 @@ def rhf(engine, nelec=None, maxiter=100, etol=1e-8, dtol=1e-6, ndiis=10, increme
             F = F + engine.fock2e(P - P_prev)       # F_prev already holds H + G(P_prev)
         else:
             F = H + engine.fock2e(P)
-        ftimes.append(time.perf_counter() - tf)
+        ftimes.append(time.perf_counter() - tf)            # build + DIIS row, to the one host sync
         E = 0.5 * float(np.sum(P * (H + F))) + enuc
         err = X.T @ (F @ P @ S - S @ P @ F) @ X
         fs.append(F)
@@ -104,3 +104,111 @@ def seeded_density(S, nocc, seed=20261202):
     w, U = np.linalg.eigh(C.T @ S @ C)
     C = C @ (U @ np.diag(w ** -0.5) @ U.T)
     return 2.0 * C @ C.T
+
+
+def rhf_device(engine, nelec=None, maxiter=100, etol=1e-8, dtol=1e-6, ndiis=10, incremental=True, ncyc=3,
+               rebuild=8, one_electron=None, device=None, verbose=False):
+    """RHF with every matrix on the GPU (torch FP64 tensors): Fock builds add
+    G into the device F (``qf_fock_build_device``), DIIS and the eigensolver
+    run on the device, and one small host transfer per iteration carries the
+    energy, the DIIS overlaps and the density change.
+
+    With ``incremental`` the build after ``ncyc`` full ones is QUICK's
+    deltaO scheme (quick_scf_operator_module.f90:69-79): F = F_prev + G(P -
+    P_prev), where the density screen sees |P - P_prev| and drops most
+    quartets late in the SCF.  A full build is redone every ``rebuild``
+    iterations and once more at convergence, and the SCF continues if that
+    full-build energy moved by more than ``etol``."""
+    import torch
+    dev = device or torch.device("cuda:0")
+    mol = engine.basis.mol
+    if nelec is None:
+        nelec = mol.nelec()
+    if nelec % 2:
+        raise ValueError("RHF needs an even number of electrons")
+    nocc = nelec // 2
+    t0 = time.perf_counter()
+    S, T, V = one_electron if one_electron is not None else engine.one_electron()
+    dd = dict(dtype=torch.float64, device=dev)
+    S = torch.tensor(S, **dd)
+    H = torch.tensor(T + V, **dd)
+    s, U = torch.linalg.eigh(S)
+    X = (U / torch.sqrt(s)) @ U.T
+    enuc = mol.nuclear_repulsion()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def density(F):
+        _, Cp = torch.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp[:, :nocc]
+        return 2.0 * C @ C.T
+
+    P = density(H)
+    F = torch.empty_like(H)
+    dP = torch.empty_like(H)
+    P_prev = None
+    fs, es = [], []
+    B = np.zeros((0, 0))
+    E_old = 0.0
+    hist, ftimes, full_flags = [], [], []
+    it_full = 0
+    converged = False
+    E = 0.0
+    for it in range(maxiter):
+        tf = time.perf_counter()
+        full = (not incremental) or it < ncyc or P_prev is None or (it - it_full) >= rebuild
+        if full:
+            F.copy_(H)
+            engine.fock2e_device(P.data_ptr(), F.data_ptr(), stream)
+            it_full = it
+        else:
+            torch.sub(P, P_prev, out=dP)
+            engine.fock2e_device(dP.data_ptr(), F.data_ptr(), stream)   # F already holds H + G(P_prev)
+        full_flags.append(full)
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        fs.append(F.clone())
+        es.append(err)
+        if len(fs) > ndiis:
+            fs.pop(0)
+            es.pop(0)
+            B = B[1:, 1:]
+        Ed = 0.5 * torch.sum(P * (H + F))
+        row = torch.stack([Ed] + [torch.sum(err * e) for e in es]).cpu().numpy()
+        ftimes.append(time.perf_counter() - tf)            # build + DIIS row, to the one host sync
+        E = float(row[0]) + enuc
+        k = len(es)
+        Bn = np.zeros((k, k))
+        Bn[:k - 1, :k - 1] = B
+        Bn[k - 1, :] = row[1:]
+        Bn[:, k - 1] = row[1:]
+        B = Bn
+        A = -np.ones((k + 1, k + 1))
+        A[:k, :k] = B
+        A[k, k] = 0.0
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            c = np.linalg.solve(A, rhs)[:k]
+            Fd = fs[0] * float(c[0])
+            for ci, fi in zip(c[1:], fs[1:]):
+                Fd = Fd + float(ci) * fi
+        except np.linalg.LinAlgError:
+            Fd = F
+        Pn = density(Fd)
+        rms = float(torch.sqrt(torch.mean((Pn - P) ** 2)))
+        hist.append(E)
+        if verbose:
+            print(f"{it:3d} {E:.12f} {E - E_old:+.3e} rmsdP {rms:.3e} {'full' if full else 'delta'} "
+                  f"{ftimes[-1] * 1e3:.1f} ms", flush=True)
+        done = abs(E - E_old) < etol and rms < dtol
+        E_old = E
+        if done:
+            if full:
+                converged = True
+                break
+            it_full = it - rebuild          # force a full build next: confirm on the full Fock matrix
+        P_prev = P
+        P = Pn
+    torch.cuda.synchronize(dev)
+    return RHFResult(energy=E, density=P.cpu().numpy(), iterations=len(hist), history=hist, fock_times=ftimes,
+                     full_builds=int(sum(full_flags)), wall=time.perf_counter() - t0, converged=converged,
+                     e_nuc=enuc)

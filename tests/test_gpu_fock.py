@@ -149,6 +149,26 @@ def test_scf_energy_matches_oracle_config1():
     assert abs(res.energy - (-75.983996360)) < 3e-8
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 2e-8), ("mixed", 1e-6)])
+def test_device_scf_matches_host_scf(precision, tol):
+    """Device-resident SCF with incremental Fock builds (QUICK's deltaO)
+    against the host SCF with full FP64 builds, water octamer 6-31G*."""
+    announce(f"device SCF ({precision}, incremental) vs host SCF, water8")
+    from quick_amd.scf import rhf, rhf_device
+    mol = molecule("water8")
+    bs = product_basis(mol, "6-31G*")
+    e64 = _engine(bs, cutoff=1e-11, prim_limit=1e-12)
+    S, T, V = e64.one_electron()
+    ref = rhf(e64, etol=1e-9, dtol=1e-7, one_electron=(S, T, V))
+    e = _engine(bs, cutoff=1e-11, prim_limit=1e-12, precision=precision)
+    res = rhf_device(e, etol=1e-9, dtol=1e-7, one_electron=(S, T, V))
+    print(f"  host {ref.energy:.10f} ({ref.iterations} it)  device {res.energy:.10f} ({res.iterations} it, "
+          f"{res.full_builds} full builds)  dE {abs(res.energy - ref.energy):.1e}", flush=True)
+    assert res.converged and ref.converged
+    assert abs(res.energy - ref.energy) <= tol
+    assert res.full_builds < res.iterations
+
+
 def test_mixed_precision_error_is_bounded():
     announce("mixed precision water8 6-31G*")
     mol = molecule("water8")
