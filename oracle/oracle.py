@@ -53,6 +53,10 @@ def lib():
         L.qo_get2e_range.restype = ctypes.c_long
         L.qo_get2e_range.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
+        L.qo_density_cutoff_os.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.qo_get2e_os.restype = ctypes.c_long
+        L.qo_get2e_os.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, dp, dp]
         L.qo_fmt.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.qo_one_electron.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.qo_eri_tensor.argtypes = [ctypes.c_void_p, dp]
@@ -243,6 +247,26 @@ class Oracle:
             nq = lib().qo_get2e_omp(self.h, _p(P), cmp, cutoff, prim_limit, hyb, threads, _p(o))
         return o, nq
 
+    def density_cutoff_os(self, Pa, Pb):
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        cm = np.zeros((self.qb.nshell, self.qb.nshell))
+        lib().qo_density_cutoff_os(self.h, _p(Pa), _p(Pb), _p(cm))
+        return cm
+
+    def fock2e_os(self, Pa, Pb, cutoff=1e-12, prim_limit=None, hyb=1.0, threads=1, cutmatrix=None):
+        """Open-shell G: (Ga, Gb, quartets) with Ga = J(Pa+Pb) - hyb K(Pa),
+        Gb = J(Pa+Pb) - hyb K(Pb) (getOshellEri)."""
+        if prim_limit is None:
+            prim_limit = 0.1 * cutoff
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        oa, ob = np.zeros((self.n, self.n)), np.zeros((self.n, self.n))
+        cm = None if cutmatrix is None else np.ascontiguousarray(cutmatrix, dtype=np.float64)
+        cmp = _p(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
+        nq = lib().qo_get2e_os(self.h, _p(Pa), _p(Pb), cmp, cutoff, prim_limit, hyb, threads, _p(oa), _p(ob))
+        return oa, ob, nq
+
     def fock2e_shell_range(self, P, ii_begin, ii_end, cutoff=1e-12, prim_limit=None, hyb=1.0):
         if prim_limit is None:
             prim_limit = 0.1 * cutoff
@@ -336,3 +360,57 @@ def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=
         E_old = E
         P = Pn
     raise RuntimeError("SCF did not converge")
+
+
+def uhf(oracle: Oracle, nalpha, nbeta, cutoff=1e-10, prim_limit=None, maxiter=200, etol=1e-10,
+        dtol=1e-8, ndiis=10, verbose=False):
+    """Plain-DIIS UHF with full open-shell builds by the oracle: F_s = H + G_s,
+    E = 1/2 sum_s P_s (H + F_s) + E_nuc (get_oshell_eri_energy, eri.fh:1062-1080),
+    one DIIS over the stacked alpha and beta error vectors, core guess."""
+    qb = oracle.qb
+    S, T, V = oracle.one_electron()
+    H = T + V
+    s, U = np.linalg.eigh(S)
+    X = U @ np.diag(s ** -0.5) @ U.T
+    enuc = nuclear_repulsion(qb.charge, qb.xyz)
+
+    def density(F, nocc):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        return C[:, :nocc] @ C[:, :nocc].T
+
+    Pa, Pb = density(H, nalpha), density(H, nbeta)
+    E_old = 0.0
+    fs, es, hist = [], [], []
+    for it in range(maxiter):
+        Ga, Gb, _ = oracle.fock2e_os(Pa, Pb, cutoff=cutoff, prim_limit=prim_limit)
+        Fa, Fb = H + Ga, H + Gb
+        E = 0.5 * (np.sum(Pa * (H + Fa)) + np.sum(Pb * (H + Fb))) + enuc
+        err = np.concatenate([(X.T @ (F @ P @ S - S @ P @ F) @ X).ravel() for F, P in ((Fa, Pa), (Fb, Pb))])
+        fs.append((Fa, Fb))
+        es.append(err)
+        fs, es = fs[-ndiis:], es[-ndiis:]
+        k = len(fs)
+        B = -np.ones((k + 1, k + 1))
+        B[k, k] = 0.0
+        for i in range(k):
+            for j in range(k):
+                B[i, j] = np.dot(es[i], es[j])
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            coef = np.linalg.solve(B, rhs)[:k]
+            Fda = sum(c * f[0] for c, f in zip(coef, fs))
+            Fdb = sum(c * f[1] for c, f in zip(coef, fs))
+        except np.linalg.LinAlgError:
+            Fda, Fdb = Fa, Fb
+        Pna, Pnb = density(Fda, nalpha), density(Fdb, nbeta)
+        dP = max(np.sqrt(np.mean((Pna - Pa) ** 2)), np.sqrt(np.mean((Pnb - Pb) ** 2)))
+        hist.append(E)
+        if verbose:
+            print(f"{it:3d} {E:.12f} {E - E_old:.3e} {dP:.3e}")
+        if abs(E - E_old) < etol and dP < dtol:
+            return E, (Pa, Pb), hist
+        E_old = E
+        Pa, Pb = Pna, Pnb
+    raise RuntimeError("UHF did not converge")

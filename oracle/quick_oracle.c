@@ -566,14 +566,91 @@ void qo_density_cutoff(const qo_ctx *c, const double *P, double *cutmatrix)
 }
 
 /* ------------------------------------------------------------------------- */
-/* Digestion of one sub-class: iclass_cshell (eri.fh:448-1057), closed shell */
+/* Digestion of one sub-class: iclass_cshell / iclass_oshell                 */
+/* (eri.fh:448-1057).  One target adds J(PJ) - hk/2 K(PK) into o:            */
+/*   closed shell: PJ = PK = P,  hk = hyb                                     */
+/*   open shell, alpha: PJ = Pa + Pb, PK = Pa, hk = 2 hyb; beta with Pb.      */
+/* Every OSHELL branch of eri.fh is the closed-shell branch with the Coulomb */
+/* densities replaced by dense + denseb and the exchange densities by dense  */
+/* (or denseb) at twice the weight, so one restatement serves both.          */
 /* ------------------------------------------------------------------------- */
-#define O_(r, cc) o[(size_t)(r) * n + (cc)]
-#define P_(r, cc) P[(size_t)(r) * n + (cc)]
+typedef struct {
+    const double *PJ, *PK;
+    double hk;
+    double *o;
+} digest_t;
 
-static void iclass_cshell(const qo_ctx *c, int II, int JJ, int KK, int LL, int I, int J, int K, int L,
-                          const prim_store *ps, const double *P, double hyb, double *o,
-                          double store[NCMAX][NCMAX])
+#define O_(r, cc) o[(size_t)(r) * n + (cc)]
+#define PJ_(r, cc) PJ[(size_t)(r) * n + (cc)]
+#define PK_(r, cc) PK[(size_t)(r) * n + (cc)]
+
+/* III < JJJ, KKK < LLL, III < KKK (eri.fh:649-704 and 719-766) */
+static void dig_general(const digest_t *d, int n, int I, int J, int K, int L, double Y)
+{
+    const double *PJ = d->PJ, *PK = d->PK, h = d->hk;
+    double *o = d->o;
+    O_(J, I) += 2.0 * PJ_(L, K) * Y;
+    O_(L, K) += 2.0 * PJ_(J, I) * Y;
+    O_(K, I) -= h * 0.5 * PK_(L, J) * Y;
+    O_(L, I) -= h * 0.5 * PK_(K, J) * Y;
+    O_(J, K) -= h * 0.5 * PK_(L, I) * Y;
+    O_(J, L) -= h * 0.5 * PK_(K, I) * Y;
+    O_(K, J) -= h * 0.5 * PK_(L, I) * Y;
+    O_(L, J) -= h * 0.5 * PK_(K, I) * Y;
+}
+
+/* III < KKK, the special index patterns (eri.fh:716-904) */
+static void dig_lt(const digest_t *d, int n, int I, int J, int K, int L, double Y)
+{
+    const double *PJ = d->PJ, *PK = d->PK, h = d->hk;
+    double *o = d->o;
+    if (I < J && K < L) {
+        dig_general(d, n, I, J, K, L, Y);
+    } else if (I == J && K == L) {
+        O_(I, I) += PJ_(K, K) * Y;
+        O_(K, K) += PJ_(I, I) * Y;
+        O_(K, I) -= h * 0.5 * PK_(K, I) * Y;
+    } else if (J == K && J == L) {
+        O_(J, I) += PJ_(J, J) * Y - h * 0.5 * PK_(J, J) * Y;
+        O_(J, J) += 2.0 * PJ_(J, I) * Y - h * PK_(J, I) * Y;
+    } else if (K == L && I < J && J != K) {
+        O_(J, I) += PJ_(K, K) * Y;
+        O_(K, K) += 2.0 * PJ_(J, I) * Y;
+        O_(K, I) -= h * 0.5 * PK_(K, J) * Y;
+        O_(K, J) -= h * 0.5 * PK_(K, I) * Y;
+        O_(J, K) -= h * 0.5 * PK_(K, I) * Y;
+    } else if (I == J && K < L) {
+        O_(L, K) += PJ_(I, I) * Y;
+        O_(I, I) += 2.0 * PJ_(L, K) * Y;
+        O_(K, I) -= h * 0.5 * PK_(L, I) * Y;
+        O_(L, I) -= h * 0.5 * PK_(K, I) * Y;
+    }
+}
+
+/* III == KKK, JJJ <= LLL (eri.fh:906-1046) */
+static void dig_eq(const digest_t *d, int n, int I, int J, int K, int L, double Y)
+{
+    const double *PJ = d->PJ, *PK = d->PK, h = d->hk;
+    double *o = d->o;
+    if (I == J && I == K && I == L) {
+        O_(I, I) += PJ_(I, I) * Y - h * 0.5 * PK_(I, I) * Y;
+    } else if (I == J && I == K && I < L) {
+        O_(L, I) += PJ_(I, I) * Y - h * 0.5 * PK_(I, I) * Y;
+        O_(I, I) += 2.0 * PJ_(L, I) * Y - h * PK_(L, I) * Y;
+    } else if (I == K && J == L && I < J) {
+        O_(J, I) += 2.0 * PJ_(J, I) * Y - h * 0.5 * PK_(J, I) * Y;
+        O_(J, J) -= h * 0.5 * PK_(I, I) * Y;
+        O_(I, I) -= h * 0.5 * PK_(J, J) * Y;
+    } else if (I == K && I < J && J < L) {
+        O_(J, I) += 2.0 * PJ_(L, I) * Y - h * 0.5 * PK_(L, I) * Y;
+        O_(L, I) += 2.0 * PJ_(J, I) * Y - h * 0.5 * PK_(J, I) * Y;
+        O_(I, I) -= h * 1.0 * PK_(L, J) * Y;
+        O_(L, J) -= h * 0.5 * PK_(I, I) * Y;
+    }
+}
+
+static void iclass(const qo_ctx *c, int II, int JJ, int KK, int LL, int I, int J, int K, int L,
+                   const prim_store *ps, const digest_t *dg, int ndg, double store[NCMAX][NCMAX])
 {
     const int n = c->nbasis;
     contract(c, ps, I, J, K, L, store);
@@ -585,7 +662,6 @@ static void iclass_cshell(const qo_ctx *c, int II, int JJ, int KK, int LL, int I
     int JJJ1 = c->ksumtype[JJ] + c->qsbasis[JJ * 4 + J], JJJ2 = c->ksumtype[JJ] + c->qfbasis[JJ * 4 + J];
     int KKK1 = c->ksumtype[KK] + c->qsbasis[KK * 4 + K], KKK2 = c->ksumtype[KK] + c->qfbasis[KK * 4 + K];
     int LLL1 = c->ksumtype[LL] + c->qsbasis[LL * 4 + L], LLL2 = c->ksumtype[LL] + c->qfbasis[LL * 4 + L];
-    const double h = hyb;
 
     if (II < JJ && II < KK && KK < LL) {
         for (int III = III1; III <= III2; ++III)
@@ -593,16 +669,7 @@ static void iclass_cshell(const qo_ctx *c, int II, int JJ, int KK, int LL, int I
                 for (int KKK = KKK1; KKK <= KKK2; ++KKK)
                     for (int LLL = LLL1; LLL <= LLL2; ++LLL) {
                         double Y = hrr_whole(c, III, JJJ, KKK, LLL, AB, CD, store);
-                        double DKI = P_(KKK, III), DKJ = P_(KKK, JJJ), DLJ = P_(LLL, JJJ);
-                        double DLI = P_(LLL, III), DLK = P_(LLL, KKK), DJI = P_(JJJ, III);
-                        O_(JJJ, III) += 2.0 * DLK * Y;
-                        O_(LLL, KKK) += 2.0 * DJI * Y;
-                        O_(KKK, III) -= h * 0.5 * DLJ * Y;
-                        O_(LLL, III) -= h * 0.5 * DKJ * Y;
-                        O_(JJJ, KKK) -= h * 0.5 * DLI * Y;
-                        O_(JJJ, LLL) -= h * 0.5 * DKI * Y;
-                        O_(KKK, JJJ) -= h * 0.5 * DLI * Y;
-                        O_(LLL, JJJ) -= h * 0.5 * DKI * Y;
+                        for (int t = 0; t < ndg; ++t) dig_general(&dg[t], n, III, JJJ, KKK, LLL, Y);
                     }
         return;
     }
@@ -612,64 +679,10 @@ static void iclass_cshell(const qo_ctx *c, int II, int JJ, int KK, int LL, int I
                 for (int LLL = (KKK > LLL1 ? KKK : LLL1); LLL <= LLL2; ++LLL) {
                     if (III < KKK) {
                         double Y = hrr_whole(c, III, JJJ, KKK, LLL, AB, CD, store);
-                        if (III < JJJ && KKK < LLL) {
-                            double DKI = P_(KKK, III), DKJ = P_(KKK, JJJ), DLJ = P_(LLL, JJJ);
-                            double DLI = P_(LLL, III), DLK = P_(LLL, KKK), DJI = P_(JJJ, III);
-                            O_(JJJ, III) += 2.0 * DLK * Y;
-                            O_(LLL, KKK) += 2.0 * DJI * Y;
-                            O_(KKK, III) -= h * 0.5 * DLJ * Y;
-                            O_(LLL, III) -= h * 0.5 * DKJ * Y;
-                            O_(JJJ, KKK) -= h * 0.5 * DLI * Y;
-                            O_(JJJ, LLL) -= h * 0.5 * DKI * Y;
-                            O_(KKK, JJJ) -= h * 0.5 * DLI * Y;
-                            O_(LLL, JJJ) -= h * 0.5 * DKI * Y;
-                        } else if (III == JJJ && KKK == LLL) {
-                            double DJI = P_(KKK, III), DJJ = P_(KKK, KKK), DII = P_(III, III);
-                            O_(III, III) += DJJ * Y;
-                            O_(KKK, KKK) += DII * Y;
-                            O_(KKK, III) -= h * 0.5 * DJI * Y;
-                        } else if (JJJ == KKK && JJJ == LLL) {
-                            double DJI = P_(JJJ, III), DJJ = P_(JJJ, JJJ);
-                            O_(JJJ, III) += DJJ * Y - h * 0.5 * DJJ * Y;
-                            O_(JJJ, JJJ) += 2.0 * DJI * Y - h * DJI * Y;
-                        } else if (KKK == LLL && III < JJJ && JJJ != KKK) {
-                            double DKI = P_(KKK, III), DKJ = P_(KKK, JJJ), DKK = P_(KKK, KKK);
-                            double DJI = P_(JJJ, III);
-                            O_(JJJ, III) += DKK * Y;
-                            O_(KKK, KKK) += 2.0 * DJI * Y;
-                            O_(KKK, III) -= h * 0.5 * DKJ * Y;
-                            O_(KKK, JJJ) -= h * 0.5 * DKI * Y;
-                            O_(JJJ, KKK) -= h * 0.5 * DKI * Y;
-                        } else if (III == JJJ && KKK < LLL) {
-                            double DII = P_(III, III), DJI = P_(KKK, III), DKI = P_(LLL, III);
-                            double DKJ = P_(LLL, KKK);
-                            O_(LLL, KKK) += DII * Y;
-                            O_(III, III) += 2.0 * DKJ * Y;
-                            O_(KKK, III) -= h * 0.5 * DKI * Y;
-                            O_(LLL, III) -= h * 0.5 * DJI * Y;
-                        }
+                        for (int t = 0; t < ndg; ++t) dig_lt(&dg[t], n, III, JJJ, KKK, LLL, Y);
                     } else if (JJJ <= LLL) {
                         double Y = hrr_whole(c, III, JJJ, KKK, LLL, AB, CD, store);
-                        if (III == JJJ && III == KKK && III == LLL) {
-                            double DII = P_(III, III);
-                            O_(III, III) += DII * Y - h * 0.5 * DII * Y;
-                        } else if (III == JJJ && III == KKK && III < LLL) {
-                            double DJI = P_(LLL, III), DII = P_(III, III);
-                            O_(LLL, III) += DII * Y - h * 0.5 * DII * Y;
-                            O_(III, III) += 2.0 * DJI * Y - h * DJI * Y;
-                        } else if (III == KKK && JJJ == LLL && III < JJJ) {
-                            double DJI = P_(JJJ, III), DJJ = P_(JJJ, JJJ), DII = P_(III, III);
-                            O_(JJJ, III) += 2.0 * DJI * Y - h * 0.5 * DJI * Y;
-                            O_(JJJ, JJJ) -= h * 0.5 * DII * Y;
-                            O_(III, III) -= h * 0.5 * DJJ * Y;
-                        } else if (III == KKK && III < JJJ && JJJ < LLL) {
-                            double DKI = P_(LLL, III), DKJ = P_(LLL, JJJ), DII = P_(III, III);
-                            double DJI = P_(JJJ, III);
-                            O_(JJJ, III) += 2.0 * DKI * Y - h * 0.5 * DKI * Y;
-                            O_(LLL, III) += 2.0 * DJI * Y - h * 0.5 * DJI * Y;
-                            O_(III, III) -= h * 1.0 * DKJ * Y;
-                            O_(LLL, JJJ) -= h * 0.5 * DII * Y;
-                        }
+                        for (int t = 0; t < ndg; ++t) dig_eq(&dg[t], n, III, JJJ, KKK, LLL, Y);
                     }
                 }
 }
@@ -697,9 +710,11 @@ static void work_free(work_t *w)
     free(w->store);
 }
 
-/* get_cshell_eri (eri.fh:161-202) for one II, then cshell (eri.fh:209-441). */
-static long get_cshell_eri(const qo_ctx *c, int II, const double *P, const double *cm,
-                           double cutoff, double prim_limit, double hyb, double *o, work_t *w)
+/* get_cshell_eri / get_oshell_eri (eri.fh:161-202) for one II, then
+ * cshell / oshell (eri.fh:209-441): screening and primitives are shared, the
+ * digestion adds into every target. */
+static long get_eri_ii(const qo_ctx *c, int II, const double *cm, double cutoff, double prim_limit,
+                       const digest_t *dg, int ndg, work_t *w)
 {
     const int ns = c->nshell;
     long nq = 0;
@@ -723,10 +738,62 @@ static long get_cshell_eri(const qo_ctx *c, int II, const double *P, const doubl
                     for (int J = c->qstart[JJ]; J <= c->qfinal[JJ]; ++J)
                         for (int K = c->qstart[KK]; K <= c->qfinal[KK]; ++K)
                             for (int L = c->qstart[LL]; L <= c->qfinal[LL]; ++L)
-                                iclass_cshell(c, II, JJ, KK, LL, I, J, K, L, &w->ps, P, hyb, o,
-                                              w->store);
+                                iclass(c, II, JJ, KK, LL, I, J, K, L, &w->ps, dg, ndg, w->store);
             }
     }
+    return nq;
+}
+
+static void copy_sym(double *o, int n)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) o[(size_t)i * n + j] = o[(size_t)j * n + i];
+}
+
+/* The build over II in [ii_begin, ii_end) on nthreads threads (QUICK's MPI
+ * build weights II by jshell - II, mpi_setup.f90:317-322; a dynamic schedule
+ * in that order gives the same balance on threads).  Each thread digests
+ * into private copies of the targets, summed at the end. */
+static long run_get2e(const qo_ctx *c, const double *cm, double cutoff, double prim_limit,
+                      const digest_t *dg, int ndg, int ii_begin, int ii_end, int nthreads)
+{
+    const int n = c->nbasis;
+    long nq = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    if (nthreads == 1) {
+        work_t w;
+        work_init(c, &w);
+        for (int II = ii_begin; II < ii_end; ++II) nq += get_eri_ii(c, II, cm, cutoff, prim_limit, dg, ndg, &w);
+        work_free(&w);
+        return nq;
+    }
+    const size_t nn = (size_t)n * n;
+    double *part = calloc((size_t)nthreads * ndg * nn, sizeof(double));
+#pragma omp parallel num_threads(nthreads) reduction(+ : nq)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        work_t w;
+        work_init(c, &w);
+        digest_t mine[2];
+        for (int t = 0; t < ndg; ++t) {
+            mine[t] = dg[t];
+            mine[t].o = part + ((size_t)tid * ndg + t) * nn;
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int II = ii_begin; II < ii_end; ++II) nq += get_eri_ii(c, II, cm, cutoff, prim_limit, mine, ndg, &w);
+        work_free(&w);
+    }
+    for (int th = 0; th < nthreads; ++th)
+        for (int t = 0; t < ndg; ++t)
+            for (size_t k = 0; k < nn; ++k) dg[t].o[k] += part[((size_t)th * ndg + t) * nn + k];
+    free(part);
     return nq;
 }
 
@@ -740,20 +807,10 @@ long qo_get2e_range(const qo_ctx *c, const double *P, const double *cutmatrix, d
         qo_density_cutoff(c, P, cm);
         cutmatrix = cm;
     }
-    work_t w;
-    work_init(c, &w);
-    long nq = 0;
-    for (int II = ii_begin; II < ii_end; ++II)
-        nq += get_cshell_eri(c, II, P, cutmatrix, cutoff, prim_limit, hyb, o, &w);
-    work_free(&w);
+    digest_t dg = {P, P, hyb, o};
+    long nq = run_get2e(c, cutmatrix, cutoff, prim_limit, &dg, 1, ii_begin, ii_end, 1);
     free(cm);
     return nq;
-}
-
-static void copy_sym(double *o, int n)
-{
-    for (int i = 0; i < n; ++i)
-        for (int j = i + 1; j < n; ++j) o[(size_t)i * n + j] = o[(size_t)j * n + i];
 }
 
 long qo_get2e(const qo_ctx *c, const double *P, const double *cutmatrix, double cutoff,
@@ -767,41 +824,64 @@ long qo_get2e(const qo_ctx *c, const double *P, const double *cutmatrix, double 
 long qo_get2e_omp(const qo_ctx *c, const double *P, const double *cutmatrix, double cutoff,
                   double prim_limit, double hyb, int nthreads, double *o)
 {
-    const int ns = c->nshell, n = c->nbasis;
+    const int ns = c->nshell;
     double *cm = NULL;
     if (!cutmatrix) {
         cm = malloc(sizeof(double) * ns * ns);
         qo_density_cutoff(c, P, cm);
         cutmatrix = cm;
     }
-    long nq = 0;
-#ifdef _OPENMP
-    if (nthreads <= 0) nthreads = omp_get_max_threads();
-#else
-    nthreads = 1;
-#endif
-    double *part = calloc((size_t)nthreads * n * n, sizeof(double));
-    /* QUICK's MPI build weights II by (jshell - II) (mpi_setup.f90:317-322);
-     * a dynamic schedule in that order gives the same balance on threads. */
-#pragma omp parallel num_threads(nthreads) reduction(+ : nq)
-    {
-        int tid = 0;
-#ifdef _OPENMP
-        tid = omp_get_thread_num();
-#endif
-        work_t w;
-        work_init(c, &w);
-        double *mine = part + (size_t)tid * n * n;
-#pragma omp for schedule(dynamic, 1)
-        for (int II = 0; II < ns; ++II)
-            nq += get_cshell_eri(c, II, P, cutmatrix, cutoff, prim_limit, hyb, mine, &w);
-        work_free(&w);
-    }
-    for (int t = 0; t < nthreads; ++t)
-        for (size_t k = 0; k < (size_t)n * n; ++k) o[k] += part[(size_t)t * n * n + k];
-    free(part);
+    digest_t dg = {P, P, hyb, o};
+    long nq = run_get2e(c, cutmatrix, cutoff, prim_limit, &dg, 1, 0, ns, nthreads);
     free(cm);
-    copy_sym(o, n);
+    copy_sym(o, c->nbasis);
+    return nq;
+}
+
+/* oshell_dnscreen / oshell_density_cutoff (cutoff.fh:16-89):
+ * cutmatrix(II,JJ) = max over the block of max(|Pa+Pb|, |Pa|, |Pb|). */
+void qo_density_cutoff_os(const qo_ctx *c, const double *Pa, const double *Pb, double *cutmatrix)
+{
+    const int n = c->nbasis, ns = c->nshell;
+    for (int II = 0; II < ns; ++II)
+        for (int JJ = II; JJ < ns; ++JJ) {
+            int i0 = c->ksumtype[II] + c->qsbasis[II * 4 + c->qstart[II]];
+            int i1 = c->ksumtype[II] + c->qfbasis[II * 4 + c->qfinal[II]];
+            int j0 = c->ksumtype[JJ] + c->qsbasis[JJ * 4 + c->qstart[JJ]];
+            int j1 = c->ksumtype[JJ] + c->qfbasis[JJ * 4 + c->qfinal[JJ]];
+            double dn = 0.0;
+            for (int i = i0; i <= i1; ++i)
+                for (int j = j0; j <= j1; ++j) {
+                    double a = Pa[j * n + i], b = Pb[j * n + i];
+                    double d = fmax(fabs(a + b), fmax(fabs(a), fabs(b)));
+                    if (d > dn) dn = d;
+                }
+            cutmatrix[II * ns + JJ] = dn;
+            cutmatrix[JJ * ns + II] = dn;
+        }
+}
+
+/* get_oshell_eri (eri.fh:159-200, iclass_oshell :446-1055) over all II, then
+ * copySym of both matrices: oa += J(Pa+Pb) - hyb K(Pa), ob += J(Pa+Pb) - hyb K(Pb). */
+long qo_get2e_os(const qo_ctx *c, const double *Pa, const double *Pb, const double *cutmatrix, double cutoff,
+                 double prim_limit, double hyb, int nthreads, double *oa, double *ob)
+{
+    const int ns = c->nshell, n = c->nbasis;
+    const size_t nn = (size_t)n * n;
+    double *cm = NULL;
+    if (!cutmatrix) {
+        cm = malloc(sizeof(double) * ns * ns);
+        qo_density_cutoff_os(c, Pa, Pb, cm);
+        cutmatrix = cm;
+    }
+    double *Pt = malloc(sizeof(double) * nn);
+    for (size_t k = 0; k < nn; ++k) Pt[k] = Pa[k] + Pb[k];
+    digest_t dg[2] = {{Pt, Pa, 2.0 * hyb, oa}, {Pt, Pb, 2.0 * hyb, ob}};
+    long nq = run_get2e(c, cutmatrix, cutoff, prim_limit, dg, 2, 0, ns, nthreads);
+    free(Pt);
+    free(cm);
+    copy_sym(oa, n);
+    copy_sym(ob, n);
     return nq;
 }
 

@@ -1,6 +1,6 @@
 /*
  * quick_oracle.h - CPU restatement of QUICK's closed-shell two-electron Fock
- * build (getCshellEri), used ONLY as test infrastructure: by tests/, by the
+ * build (getCshellEri, and getOshellEri), used ONLY as test infrastructure: by tests/, by the
  * smoke check and by bench.py's cpu_baseline leg.  Nothing in the product
  * (quick_amd/, libquickfock) links or calls it.
  *
@@ -64,6 +64,17 @@ long qo_get2e_range(const qo_ctx *c, const double *P, const double *cutmatrix,
 long qo_get2e_omp(const qo_ctx *c, const double *P, const double *cutmatrix,
                   double integral_cutoff, double prim_limit, double hyb,
                   int nthreads, double *o);
+
+/* cutoff.fh:16-89 (oshell_density_cutoff): cutmatrix(II,JJ) = max over the
+ * shell block of max(|Pa+Pb|, |Pa|, |Pb|). */
+void qo_density_cutoff_os(const qo_ctx *c, const double *Pa, const double *Pb, double *cutmatrix);
+
+/* Open-shell build (getOshellEri, eri.fh:159-200 with iclass_oshell
+ * :446-1055): oa += J(Pa+Pb) - hyb K(Pa), ob += J(Pa+Pb) - hyb K(Pb), both
+ * symmetric.  cutmatrix NULL: computed by qo_density_cutoff_os.  nthreads
+ * as qo_get2e_omp (1: serial).  Returns the screened shell-quartet count. */
+long qo_get2e_os(const qo_ctx *c, const double *Pa, const double *Pb, const double *cutmatrix, double cutoff,
+                 double prim_limit, double hyb, int nthreads, double *oa, double *ob);
 
 /* fmt.f90:13-85: F_m(x) for m = 0..maxm into vals[0..maxm]. */
 void qo_fmt(int maxm, double x, double *vals);

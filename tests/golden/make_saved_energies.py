@@ -1,10 +1,12 @@
-"""Collect QUICK's own RHF regression fixtures into a small JSON file.
+"""Collect QUICK's own RHF and UHF regression fixtures into small JSON files.
 
 Reads the reference's ``test/ene_*_rhf_*.in`` inputs (geometry, basis,
 charge) and the matching ``test/saved/*.out`` outputs (TOTAL ENERGY,
 ELECTRONIC ENERGY, CORE_CORE REPULSION, and the SCF cycle count) and writes
-``saved_rhf_energies.json`` beside this script.  Only data is kept: inputs and
-expected outputs.
+``saved_rhf_energies.json`` beside this script.  The ``*_uhf_*.in`` inputs
+(with their multiplicity) go to ``saved_uhf_energies.json``; for a geometry
+optimisation the record is the first SCF, at the input geometry.  Only data
+is kept: inputs and expected outputs.
 
     python tests/golden/make_saved_energies.py /root/reference/test
 """
@@ -15,7 +17,7 @@ import re
 import sys
 
 BASES = {"STO-3G": "STO-3G", "3-21G": "3-21G", "6-31G": "6-31G", "6-31G*": "6-31G*",
-         "6-31G**": "6-31G**", "cc-pvdz": "cc-pVDZ"}
+         "6-31G**": "6-31G**", "cc-pvdz": "cc-pVDZ", "6-31g**": "6-31G**"}
 KEEP = {"H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si",
         "P", "S", "Cl", "Ar"}
 
@@ -28,6 +30,8 @@ def parse_in(path):
     basis = m.group(1)
     m = re.search(r"CHARGE=([+-]?\d+)", kw)
     charge = int(m.group(1)) if m else 0
+    m = re.search(r"MULT=(\d+)", kw, re.I)
+    mult = int(m.group(1)) if m else 1
     m = re.search(r"cutoff=(\S+)", kw, re.I)
     cutoff = float(m.group(1)) if m else 1e-7
     syms, xyz = [], []
@@ -39,7 +43,7 @@ def parse_in(path):
             continue
         syms.append(t[0].capitalize())
         xyz.append([float(v) for v in t[1:]])
-    return basis, charge, cutoff, syms, xyz, kw
+    return basis, charge, cutoff, syms, xyz, kw, mult
 
 
 def parse_out(path):
@@ -60,14 +64,14 @@ def parse_out(path):
     return out
 
 
-def main(test_dir):
+def collect(test_dir, pattern):
     recs = []
-    for inp in sorted(glob.glob(os.path.join(test_dir, "ene_*_rhf_*.in"))):
+    for inp in sorted(glob.glob(os.path.join(test_dir, pattern))):
         name = os.path.basename(inp)[:-3]
         outp = os.path.join(test_dir, "saved", name + ".out")
         if not os.path.exists(outp):
             continue
-        basis, charge, cutoff, syms, xyz, kw = parse_in(inp)
+        basis, charge, cutoff, syms, xyz, kw, mult = parse_in(inp)
         if basis not in BASES or not set(syms) <= KEEP:
             continue
         if "DFT" in kw.upper() or "B3LYP" in kw.upper():
@@ -75,13 +79,22 @@ def main(test_dir):
         res = parse_out(outp)
         if "E_total" not in res:
             continue
-        recs.append(dict(name=name, basis=BASES[basis], charge=charge, cutoff=cutoff,
-                         symbols=syms, xyz_angstrom=xyz, **res))
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "saved_rhf_energies.json")
-    with open(dst, "w") as fh:
-        json.dump({"source": "QUICK test/saved regression outputs (printed to 1e-9 Eh)",
-                   "records": recs}, fh, indent=1)
-    print(len(recs), "records ->", dst)
+        rec = dict(name=name, basis=BASES[basis], charge=charge, cutoff=cutoff,
+                   symbols=syms, xyz_angstrom=xyz, **res)
+        if mult != 1:
+            rec["mult"] = mult
+        recs.append(rec)
+    return recs
+
+
+def main(test_dir):
+    for pattern, fname in [("ene_*_rhf_*.in", "saved_rhf_energies.json"), ("*_uhf_*.in", "saved_uhf_energies.json")]:
+        recs = collect(test_dir, pattern)
+        dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), fname)
+        with open(dst, "w") as fh:
+            json.dump({"source": "QUICK test/saved regression outputs (printed to 1e-9 Eh)",
+                       "records": recs}, fh, indent=1)
+        print(len(recs), "records ->", dst)
 
 
 if __name__ == "__main__":
