@@ -101,6 +101,19 @@ int qf_fock_build_device(qf_engine *e, const double *d_P, double *d_F, const dou
 /* Host convenience: F[n*n] += G(P[n*n]); synchronous. */
 int qf_fock_build_host(qf_engine *e, const double *P, double *F, const double *cutmat);
 
+/* Open-shell build (QUICK's get_oshell_eri: gpu_get2e_getxc_drivers.h:25 and the
+ * OSHELL branches of gpu_get2e_subs.h:391-575):
+ *   F_a += J(P_a + P_b) - hyb K(P_a),   F_b += J(P_a + P_b) - hyb K(P_b).
+ * Screening uses QUICK's open-shell density matrix (cutoff.fh:16-89, max of
+ * |P_a + P_b|, |P_a|, |P_b| per shell pair) unless d_cutmat is given.
+ * Device pointers, asynchronous on stream; needs qf_prepare. */
+int qf_fock_build_os_device(qf_engine *e, const double *d_Pa, const double *d_Pb, double *d_Fa, double *d_Fb,
+                            const double *d_cutmat, void *stream);
+
+/* Host convenience of the open-shell build: Fa += G_a, Fb += G_b; synchronous. */
+int qf_fock_build_os_host(qf_engine *e, const double *Pa, const double *Pb, double *Fa, double *Fb,
+                          const double *cutmat);
+
 /* Device pointer of the engine's G (= Gt + Gt^T after the last build). */
 const double *qf_device_g(qf_engine *e);
 
@@ -245,6 +258,13 @@ void gpu_upload_cutoff_(double *cutMatrix, double *integralCutoff, double *primL
                         double *coreIntegralCutoff);
 /* gpu_get2e_getxc_drivers.h:27 gpu_get_cshell_eri_: o += G(dense), both triangles */
 void gpu_get_cshell_eri_(bool *deltaO, double *o);
+/* gpu.cu:1771 gpu_upload_beta_density_matrix_ (the beta density of the next open-shell build) */
+void gpu_upload_beta_density_matrix_(double *denseb);
+/* gpu.cu:1728 gpu_upload_calculated_beta_ (only denseb is read) */
+void gpu_upload_calculated_beta_(double *ob, double *denseb);
+/* gpu_get2e_getxc_drivers.h:25 gpu_get_oshell_eri_ (called from quick_uscf_operator_module.f90:128):
+ * o += J(dense + denseb) - hyb K(dense), ob += J(dense + denseb) - hyb K(denseb), both triangles */
+void gpu_get_oshell_eri_(bool *deltaO, double *o, double *ob);
 /* gpu.cu:2882 gpu_cleanup_ */
 void gpu_cleanup_(void);
 

@@ -60,6 +60,8 @@ def lib():
         "qf_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(QfStats)]),
         "qf_fock_build_device": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "qf_fock_build_host": (ctypes.c_int, [vp, dp, dp, dp]),
+        "qf_fock_build_os_device": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+        "qf_fock_build_os_host": (ctypes.c_int, [vp, dp, dp, dp, dp, dp]),
         "qf_device_g": (vp, [vp]),
         "qf_fock_build_batch": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp),
                                                ctypes.POINTER(vp), vp]),

@@ -108,6 +108,34 @@ __global__ void k_cutmat(const double* __restrict__ P, int n, int nshell, const 
     cm[idx] = m;
 }
 
+// QUICK's oshell_density_cutoff (cutoff.fh:16-89): max over the block of
+// max(|Pa+Pb|, |Pa|, |Pb|).
+__global__ void k_cutmat_os(const double* __restrict__ Pa, const double* __restrict__ Pb, int n, int nshell,
+                            const int* __restrict__ f0, const int* __restrict__ f1, double* __restrict__ cm)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nshell * nshell) return;
+    const int I = idx / nshell, J = idx % nshell;
+    double m = 0.0;
+    for (int i = f0[I]; i <= f1[I]; ++i)
+        for (int j = f0[J]; j <= f1[J]; ++j) {
+            const double a = Pa[i * n + j], b = Pb[i * n + j];
+            m = fmax(m, fmax(fabs(a + b), fmax(fabs(a), fabs(b))));
+        }
+    cm[idx] = m;
+}
+
+// Total and spin densities of the open-shell build: Pt = Pa + Pb, Ps = Pa - Pb.
+__global__ void k_spin(const double* __restrict__ Pa, const double* __restrict__ Pb, double* __restrict__ Pt,
+                       double* __restrict__ Ps, int n2)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n2) return;
+    const double a = Pa[idx], b = Pb[idx];
+    Pt[idx] = a + b;
+    Ps[idx] = a - b;
+}
+
 __global__ void k_zero(double* __restrict__ p, size_t count)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
@@ -115,9 +143,9 @@ __global__ void k_zero(double* __restrict__ p, size_t count)
 }
 
 __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* __restrict__ Gs,
-                           double* __restrict__ F, int n)
+                           double* __restrict__ F, double* __restrict__ F2, double sign2, int n)
 {
-    // G = sum over copies of (Gt + Gt^T); F += G
+    // G = sum over copies of (Gt + Gt^T); F += G, F2 += sign2 G
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
     const int i = idx / n, j = idx % n;
@@ -126,6 +154,7 @@ __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* 
     for (int r = 0; r < replicas; ++r) g += Gt[r * n2 + i * n + j] + Gt[r * n2 + j * n + i];
     Gs[idx] = g;
     if (F) F[idx] += g;
+    if (F2) F2[idx] += sign2 * g;
 }
 
 // Counts, per quartet class, the quartets and primitive quartets that pass
@@ -225,9 +254,10 @@ struct qf_engine {
     DevBuf<qf::TileGroup> d_groups;
     DevBuf<int> d_colf;
     DevBuf<double> d_Gt, d_Gs, d_cm, d_P;
+    DevBuf<double> d_Pb, d_Pt, d_Ps, d_Fa, d_Fb;   // open-shell builds (allocated on first use)
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
-    qf::MolDesc mol{};
+    qf::MolDesc mol[2]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2
     // per-launch timing (HIP events on the build stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -775,7 +805,7 @@ int prepare(qf_engine* e)
     }
     QF_HIP(e->d_f0.upload(f0));
     QF_HIP(e->d_f1.upload(f1));
-    QF_HIP(e->d_mol.resize(1));
+    QF_HIP(e->d_mol.resize(2));
     QF_HIP(e->d_counts.resize(2 * std::max<size_t>(1, e->launches.size())));
     e->stats.device_bytes = (long long)(e->d_info.bytes() + e->d_prim.bytes() + e->d_tasks.bytes() +
                                         e->d_Gt.bytes() * 3 + e->d_cm.bytes() + e->d_blocks.bytes());
@@ -783,16 +813,13 @@ int prepare(qf_engine* e)
     return QF_OK;
 }
 
-int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStream_t s)
+// One pass over every class launch: G(P) with the density screen dcm,
+// Coulomb terms on or off, then F += G and F2 += sign2 G.  `slot` picks the
+// device MolDesc, so two passes can be queued back to back on one stream.
+int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, double* dF, double* dF2, double sign2,
+               int slot, hipStream_t s)
 {
-    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
     const int n = e->nbasis, ns = e->nshell;
-    if (!dcm) {
-        const int nthr = 256, tot = ns * ns;
-        hipLaunchKernelGGL(k_cutmat, dim3((tot + nthr - 1) / nthr), dim3(nthr), 0, s, dP, n, ns, e->d_f0.p,
-                           e->d_f1.p, e->d_cm.p);
-        dcm = e->d_cm.p;
-    }
     qf::MolDesc md{};
     md.P = dP;
     md.G = e->d_Gt.p;
@@ -804,8 +831,10 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
     md.hyb = e->hyb;
     md.replicas = e->replicas;
     md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
-    e->mol = md;
-    QF_HIP(hipMemcpyAsync(e->d_mol.p, &e->mol, sizeof(md), hipMemcpyHostToDevice, s));
+    md.jon = jon;
+    e->mol[slot] = md;
+    qf::MolDesc* dmol = e->d_mol.p + slot;
+    QF_HIP(hipMemcpyAsync(dmol, &e->mol[slot], sizeof(md), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n * e->replicas);
     for (size_t li = 0; li < e->launches.size(); ++li) {
         const auto& L = e->launches[li];
@@ -827,21 +856,60 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
         if (L.tile == 2) {
             qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
             if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, e->persist, s);
+            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, e->persist, s);
         } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
-            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, e->d_mol.p, L.fp32, L.waves, L.lds,
+            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, dmol, L.fp32, L.waves, L.lds,
                e->persist, s);
         } else {
-            f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, e->d_mol.p, L.fp32, e->persist, s);
+            f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, e->persist, s);
         }
         if (e->timing) QF_HIP(hipEventRecord(eb, s));
     }
     hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, e->d_Gt.p, e->replicas, e->d_Gs.p,
-                       dF, n);
+                       dF, dF2, sign2, n);
     QF_HIP(hipGetLastError());
     return QF_OK;
+}
+
+int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStream_t s)
+{
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    const int n = e->nbasis, ns = e->nshell;
+    if (!dcm) {
+        const int nthr = 256, tot = ns * ns;
+        hipLaunchKernelGGL(k_cutmat, dim3((tot + nthr - 1) / nthr), dim3(nthr), 0, s, dP, n, ns, e->d_f0.p,
+                           e->d_f1.p, e->d_cm.p);
+        dcm = e->d_cm.p;
+    }
+    return build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s);
+}
+
+// Open-shell build (QUICK's get_oshell_eri, gpu_get2e_subs.h OSHELL
+// branches): Fa += J(Pt) - hyb K(Pa), Fb += J(Pt) - hyb K(Pb).  With the
+// spin density Ps = Pa - Pb this is two passes of the closed-shell kernels:
+//   A = J(Pt) - hyb/2 K(Pt)   (the closed-shell G of Pt)
+//   B = -hyb/2 K(Ps)          (exchange terms only)
+// and Fa += A + B, Fb += A - B.  Both passes screen with QUICK's open-shell
+// density matrix, so they see exactly the quartets QUICK's kernel sees.
+int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, double* dFb, const double* dcm,
+             hipStream_t s)
+{
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    const int n = e->nbasis, ns = e->nshell, n2 = n * n;
+    QF_HIP(e->d_Pt.resize(n2));
+    QF_HIP(e->d_Ps.resize(n2));
+    hipLaunchKernelGGL(k_spin, dim3((n2 + 255) / 256), dim3(256), 0, s, dPa, dPb, e->d_Pt.p, e->d_Ps.p, n2);
+    if (!dcm) {
+        const int nthr = 256, tot = ns * ns;
+        hipLaunchKernelGGL(k_cutmat_os, dim3((tot + nthr - 1) / nthr), dim3(nthr), 0, s, dPa, dPb, n, ns, e->d_f0.p,
+                           e->d_f1.p, e->d_cm.p);
+        dcm = e->d_cm.p;
+    }
+    int rc = build_pass(e, e->d_Pt.p, dcm, 1, dFa, dFb, 1.0, 0, s);
+    if (rc) return rc;
+    return build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s);
 }
 
 }  // namespace
@@ -998,6 +1066,42 @@ int qf_fock_build_host(qf_engine* e, const double* P, double* F, const double* c
     std::vector<double> G(n2);
     QF_HIP(hipMemcpy(G.data(), e->d_Gs.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < n2; ++k) F[k] += G[k];
+    return QF_OK;
+}
+
+int qf_fock_build_os_device(qf_engine* e, const double* d_Pa, const double* d_Pb, double* d_Fa, double* d_Fb,
+                            const double* d_cutmat, void* stream)
+{
+    if (!e || !d_Pa || !d_Pb || !d_Fa || !d_Fb) return fail(QF_ERR_ARG, "NULL");
+    int rc = e->set_device();
+    if (rc) return rc;
+    return build_os(e, d_Pa, d_Pb, d_Fa, d_Fb, d_cutmat, (hipStream_t)stream);
+}
+
+int qf_fock_build_os_host(qf_engine* e, const double* Pa, const double* Pb, double* Fa, double* Fb,
+                          const double* cutmat)
+{
+    if (!e || !Pa || !Pb || !Fa || !Fb) return fail(QF_ERR_ARG, "NULL");
+    int rc = e->set_device();
+    if (rc) return rc;
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    const size_t n2 = (size_t)e->nbasis * e->nbasis;
+    QF_HIP(e->d_Pb.resize(n2));
+    QF_HIP(e->d_Fa.resize(n2));
+    QF_HIP(e->d_Fb.resize(n2));
+    QF_HIP(hipMemcpy(e->d_P.p, Pa, n2 * sizeof(double), hipMemcpyHostToDevice));
+    QF_HIP(hipMemcpy(e->d_Pb.p, Pb, n2 * sizeof(double), hipMemcpyHostToDevice));
+    QF_HIP(hipMemcpy(e->d_Fa.p, Fa, n2 * sizeof(double), hipMemcpyHostToDevice));
+    QF_HIP(hipMemcpy(e->d_Fb.p, Fb, n2 * sizeof(double), hipMemcpyHostToDevice));
+    const double* dcm = nullptr;
+    if (cutmat) {
+        QF_HIP(hipMemcpy(e->d_cm.p, cutmat, (size_t)e->nshell * e->nshell * sizeof(double), hipMemcpyHostToDevice));
+        dcm = e->d_cm.p;
+    }
+    rc = build_os(e, e->d_P.p, e->d_Pb.p, e->d_Fa.p, e->d_Fb.p, dcm, 0);
+    if (rc) return rc;
+    QF_HIP(hipMemcpy(Fa, e->d_Fa.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
+    QF_HIP(hipMemcpy(Fb, e->d_Fb.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
     return QF_OK;
 }
 
@@ -1373,6 +1477,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         m.cutoff = e->cutoff;
         m.primlim = e->primlim;
         m.hyb = e->hyb;
+        m.jon = 1;
         m.replicas = replicas;
         m.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
         md.push_back(m);

@@ -190,7 +190,7 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
     constexpr int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
                   o6 = o5 + nb * nd;
 #define IX(a, b, c, d) Ic[(((a) * nb + (b)) * nc + (c)) * nd + (d)]
-    for (int o = lane; o < o6; o += D::NT) {
+    for (int o = (md.jon ? 0 : o2) + lane; o < o6; o += D::NT) {   // jon = 0: exchange blocks only
         double v = 0.0;
         int row, col;
         if (o < o1) {                              // J_ab = sum_cd D_cd I

@@ -243,7 +243,8 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 //   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc I, Gt_bc -= .. D_ad I, Gt_bd -= .. D_ac I,
 // and G = Gt + Gt^T afterwards.  Because a is fixed for the wave, J_cd, K_ac
 // and K_ad stay in registers across the bra loop and J_ab is summed over
-// the wave per bra.
+// the wave per bra.  With md.jon = 0 the Coulomb terms are skipped
+// (the exchange-only pass of the open-shell build).
 //
 // TILE = false: everything goes straight to G with FP64 atomics (K_bc and
 // K_bd per quartet).  TILE = true: J_ab and every exchange term go to the
@@ -370,18 +371,20 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         Kbd[b * nd + d] += Dac[a * nc + c] * v;
                     }
         // bra block: summed over the wave
+        if (md.jon) {
 #pragma unroll
-        for (int a = 0; a < na; ++a)
+            for (int a = 0; a < na; ++a)
 #pragma unroll
-            for (int b = 0; b < nb; ++b) {
-                const double v = wave_sum(Jab[a * nb + b]);
-                if (lane == 0 && v != 0.0) {
-                    if (TILE)
-                        lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
-                    else
-                        atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+                for (int b = 0; b < nb; ++b) {
+                    const double v = wave_sum(Jab[a * nb + b]);
+                    if (lane == 0 && v != 0.0) {
+                        if (TILE)
+                            lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
+                        else
+                            atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
+                    }
                 }
-            }
+        }
         if (valid) {
             if (TILE) {
                 const int r0 = na + (kk - tr.chunk_kb) * nb;
@@ -407,10 +410,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         }
     }
     if (!touched) return;
+    if (md.jon) {
 #pragma unroll
-    for (int c = 0; c < nc; ++c)
+        for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
+            for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
+    }
     if (TILE) {
 #pragma unroll
         for (int a = 0; a < na; ++a) {

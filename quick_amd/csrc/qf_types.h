@@ -44,7 +44,7 @@ struct MolDesc {
     double cutoff, primlim, hyb;
     double split;           // quartets with Ycut*Ycut >= split run in FP64, below in FP32
     int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
-    int pad;
+    int jon;                // 1: Coulomb and exchange terms; 0: exchange terms only
 };
 
 // One wave: bras bperm[kb..ke) of one bra block - all with the same first

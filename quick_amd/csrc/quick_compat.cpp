@@ -29,7 +29,7 @@ struct QuickState {
     bool cut_set = false;
     bool oshell = false;
     int nshell = 0, jbasis = 0;
-    std::vector<double> dense, cutmat;
+    std::vector<double> dense, denseb, cutmat;
     std::vector<double> xyz, chg;          // atoms (bohr) and nuclear charges
     std::vector<double> ext_xyz, ext_chg;  // external point charges (gpu_upload_oei_)
 };
@@ -142,6 +142,12 @@ void gpu_upload_density_matrix_(double* dense)
     g_state->dense.assign(dense, dense + (size_t)g_state->nbasis * g_state->nbasis);
 }
 
+void gpu_upload_beta_density_matrix_(double* denseb)
+{
+    if (!g_state) return;
+    g_state->denseb.assign(denseb, denseb + (size_t)g_state->nbasis * g_state->nbasis);
+}
+
 void gpu_get_device_info_(int* gpu_dev_count, int* gpu_dev_id, int* gpu_dev_mem, int* gpu_num_proc,
                           double* gpu_core_freq, char* gpu_dev_name, int* name_len, char* gpu_arch_name,
                           int* arch_len, int* majorv, int* minorv, int* ierr)
@@ -223,6 +229,12 @@ void gpu_upload_calculated_(double* /*o*/, double* /*co*/, double* /*vec*/, doub
     std::memcpy(g_state->dense.data(), dense, g_state->dense.size() * sizeof(double));
 }
 
+void gpu_upload_calculated_beta_(double* /*ob*/, double* denseb)
+{
+    // only denseb is read: ob is added to by gpu_get_oshell_eri_
+    gpu_upload_beta_density_matrix_(denseb);
+}
+
 void gpu_upload_cutoff_(double* cutMatrix, double* integralCutoff, double* primLimit, double* /*DMCutoff*/,
                         double* /*coreIntegralCutoff*/)
 {
@@ -245,7 +257,7 @@ void gpu_get_cshell_eri_(bool* /*deltaO*/, double* o)
         std::exit(EXIT_FAILURE);
     }
     if (g_state->oshell) {
-        std::fprintf(stderr, "quickfock: open-shell builds are not supported\n");
+        std::fprintf(stderr, "quickfock: gpu_get_cshell_eri_ called for an open-shell method (use gpu_get_oshell_eri_)\n");
         std::exit(EXIT_FAILURE);
     }
     if (!g_state->cut_set) {
@@ -256,6 +268,29 @@ void gpu_get_cshell_eri_(bool* /*deltaO*/, double* o)
     if (qf_prepare(g_state->eng) != QF_OK) fatal("gpu_get_cshell_eri_ (prepare)");
     const double* cm = g_state->cutmat.empty() ? nullptr : g_state->cutmat.data();
     if (qf_fock_build_host(g_state->eng, g_state->dense.data(), o, cm)) fatal("gpu_get_cshell_eri_");
+}
+
+void gpu_get_oshell_eri_(bool* /*deltaO*/, double* o, double* ob)
+{
+    // o += J(dense + denseb) - hyb K(dense), ob += J(dense + denseb) - hyb K(denseb)
+    // (quick_uscf_operator_module.f90:96-128); with deltaO the incremental
+    // densities arrive in dense and denseb, as for the closed shell.
+    if (!g_state || !g_state->eng) {
+        std::fprintf(stderr, "quickfock: gpu_get_oshell_eri_ called before gpu_new_/gpu_upload_basis_\n");
+        std::exit(EXIT_FAILURE);
+    }
+    if (g_state->denseb.size() != g_state->dense.size()) {
+        std::fprintf(stderr, "quickfock: gpu_get_oshell_eri_ called before gpu_upload_calculated_beta_\n");
+        std::exit(EXIT_FAILURE);
+    }
+    if (!g_state->cut_set) {
+        qf_set_cutoffs(g_state->eng, g_state->cutoff, g_state->primlim, g_state->hyb);
+        g_state->cut_set = true;
+    }
+    if (qf_prepare(g_state->eng) != QF_OK) fatal("gpu_get_oshell_eri_ (prepare)");
+    const double* cm = g_state->cutmat.empty() ? nullptr : g_state->cutmat.data();
+    if (qf_fock_build_os_host(g_state->eng, g_state->dense.data(), g_state->denseb.data(), o, ob, cm))
+        fatal("gpu_get_oshell_eri_");
 }
 
 void gpu_cleanup_(void) {}

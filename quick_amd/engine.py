@@ -113,6 +113,26 @@ class FockEngine:
                                               ctypes.c_void_p(cutmat_ptr or 0), ctypes.c_void_p(stream or 0)),
                 "qf_fock_build_device")
 
+    def fock2e_os(self, Pa, Pb, cutmatrix=None):
+        """Open-shell G for host densities (synchronous): (Ga, Gb) with
+        Ga = J(Pa+Pb) - hyb K(Pa), Gb = J(Pa+Pb) - hyb K(Pb)."""
+        self.prepare()
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        Ga, Gb = np.zeros((self.n, self.n)), np.zeros((self.n, self.n))
+        cm = None if cutmatrix is None else np.ascontiguousarray(cutmatrix, dtype=np.float64)
+        cmp = L.dptr(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
+        L.check(self.lib.qf_fock_build_os_host(self.h, L.dptr(Pa), L.dptr(Pb), L.dptr(Ga), L.dptr(Gb), cmp),
+                "qf_fock_build_os_host")
+        return Ga, Gb
+
+    def fock2e_os_device(self, Pa_ptr, Pb_ptr, Fa_ptr, Fb_ptr, stream=None, cutmat_ptr=None):
+        """Fa += Ga, Fb += Gb with device pointers (ints); asynchronous."""
+        self.prepare()
+        v = ctypes.c_void_p
+        L.check(self.lib.qf_fock_build_os_device(self.h, v(Pa_ptr), v(Pb_ptr), v(Fa_ptr), v(Fb_ptr),
+                                                 v(cutmat_ptr or 0), v(stream or 0)), "qf_fock_build_os_device")
+
     def device_g_ptr(self):
         return self.lib.qf_device_g(self.h)
 

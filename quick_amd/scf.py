@@ -11,6 +11,11 @@ Restates the control flow of QUICK's ``electdiis`` (quick_scf_module.f90:
   system is singular (quick_scf_module.f90:405-602),
 * X = S^-1/2, P = 2 C_occ C_occ^T.
 
+``uhf`` is the unrestricted counterpart (QUICK's uscf / uscf_operator,
+quick_uscf_operator_module.f90:26-250): F_a = H + J(P_a + P_b) - K(P_a),
+F_b likewise, one DIIS over the stacked alpha and beta error vectors, and
+E = 1/2 sum_s P_s (H + F_s) + E_nuc (get_oshell_eri_energy, eri.fh:1062-1080).
+
 The Fock build runs on the GPU through :class:`quick_amd.engine.FockEngine`;
 diagonalisation and DIIS (O(n^3), FP64) use LAPACK on the host.
 """
@@ -60,7 +65,7 @@ def rhf(engine, nelec=None, maxiter=100, etol=1e-8, dtol=1e-6, ndiis=10, increme
             F = F + engine.fock2e(P - P_prev)       # F_prev already holds H + G(P_prev)
         else:
             F = H + engine.fock2e(P)
-        ftimes.append(time.perf_counter() - tf)            # build + DIIS row, to the one host sync
+        ftimes.append(time.perf_counter() - tf)
         E = 0.5 * float(np.sum(P * (H + F))) + enuc
         err = X.T @ (F @ P @ S - S @ P @ F) @ X
         fs.append(F)
@@ -93,6 +98,81 @@ def rhf(engine, nelec=None, maxiter=100, etol=1e-8, dtol=1e-6, ndiis=10, increme
                              wall=time.perf_counter() - t0, converged=True, e_nuc=enuc)
     return RHFResult(energy=E, density=P_prev, iterations=maxiter, history=hist, fock_times=ftimes,
                      wall=time.perf_counter() - t0, converged=False, e_nuc=enuc)
+
+
+def uhf(engine, mult=None, nelec=None, maxiter=200, etol=1e-8, dtol=1e-6, ndiis=10, incremental=False, ncyc=3,
+        one_electron=None, verbose=False):
+    """Unrestricted HF with the open-shell GPU build (``FockEngine.fock2e_os``).
+    n_alpha - n_beta = mult - 1; core-Hamiltonian guess.  With ``incremental``
+    the builds after ``ncyc`` full ones add G(dPa, dPb) to the previous Fock
+    matrices (deltaO, quick_uscf_operator_module.f90:71-86)."""
+    mol = engine.basis.mol
+    if nelec is None:
+        nelec = mol.nelec()
+    if mult is None:
+        mult = 1 + nelec % 2
+    if (nelec + mult - 1) % 2 or mult < 1 or mult - 1 > nelec:
+        raise ValueError("multiplicity does not fit the electron count")
+    na = (nelec + mult - 1) // 2
+    nb = nelec - na
+    t0 = time.perf_counter()
+    S, T, V = one_electron if one_electron is not None else engine.one_electron()
+    H = T + V
+    X = _orth(S)
+    enuc = mol.nuclear_repulsion()
+
+    def density(F, nocc):
+        _, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp[:, :nocc]
+        return C @ C.T
+
+    Pa, Pb = density(H, na), density(H, nb)
+    fs, es, hist, ftimes = [], [], [], []
+    E_old = 0.0
+    Fa = Fb = None
+    prev = None
+    for it in range(maxiter):
+        tf = time.perf_counter()
+        if incremental and it >= ncyc and Fa is not None:
+            Ga, Gb = engine.fock2e_os(Pa - prev[0], Pb - prev[1])
+            Fa, Fb = Fa + Ga, Fb + Gb
+        else:
+            Ga, Gb = engine.fock2e_os(Pa, Pb)
+            Fa, Fb = H + Ga, H + Gb
+        ftimes.append(time.perf_counter() - tf)
+        E = 0.5 * float(np.sum(Pa * (H + Fa)) + np.sum(Pb * (H + Fb))) + enuc
+        err = np.concatenate([(X.T @ (F @ P @ S - S @ P @ F) @ X).ravel() for F, P in ((Fa, Pa), (Fb, Pb))])
+        fs.append((Fa, Fb))
+        es.append(err)
+        fs, es = fs[-ndiis:], es[-ndiis:]
+        k = len(fs)
+        B = -np.ones((k + 1, k + 1))
+        B[k, k] = 0.0
+        for i in range(k):
+            for j in range(k):
+                B[i, j] = float(np.dot(es[i], es[j]))
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:k]
+            Fda = sum(ci * f[0] for ci, f in zip(c, fs))
+            Fdb = sum(ci * f[1] for ci, f in zip(c, fs))
+        except np.linalg.LinAlgError:
+            Fda, Fdb = Fa, Fb
+        Pna, Pnb = density(Fda, na), density(Fdb, nb)
+        dP = max(float(np.sqrt(np.mean((Pna - Pa) ** 2))), float(np.sqrt(np.mean((Pnb - Pb) ** 2))))
+        hist.append(E)
+        if verbose:
+            print(f"{it:3d} {E:.12f} {E - E_old:+.3e} rmsdP {dP:.3e} fock {ftimes[-1] * 1e3:.1f} ms", flush=True)
+        converged = abs(E - E_old) < etol and dP < dtol
+        E_old = E
+        prev = (Pa, Pb)
+        Pa, Pb = Pna, Pnb
+        if converged:
+            return RHFResult(energy=E, density=prev, iterations=it + 1, history=hist, fock_times=ftimes,
+                             wall=time.perf_counter() - t0, converged=True, e_nuc=enuc, nalpha=na, nbeta=nb)
+    return RHFResult(energy=E, density=prev, iterations=maxiter, history=hist, fock_times=ftimes,
+                     wall=time.perf_counter() - t0, converged=False, e_nuc=enuc, nalpha=na, nbeta=nb)
 
 
 def seeded_density(S, nocc, seed=20261202):

@@ -8,7 +8,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from tests.helpers import announce, molecule, oracle_for, product_basis, seeded_density_for
+from tests.helpers import announce, molecule, oracle_for, product_basis, quick_abi_setup, seeded_density_for
 
 pytestmark = pytest.mark.gpu
 
@@ -221,46 +221,11 @@ def test_quick_fortran_abi_drop_in():
     qb = orc.qb
     P, _ = seeded_density_for(orc, mol)
     G_ref, _ = orc.fock2e(P, cutoff=1e-12)
-    n, ns, jb = qb.nbasis, qb.nshell, qb.jbasis
-    I = lambda v: ctypes.byref(ctypes.c_int(int(v)))
-    D = lambda v: ctypes.byref(ctypes.c_double(float(v)))
-    ierr = ctypes.c_int(0)
-    lib.gpu_new_(ctypes.byref(ierr))
-    lib.gpu_set_device_(I(0), ctypes.byref(ierr))
-    lib.gpu_init_device_(ctypes.byref(ierr))
-    assert ierr.value == 0
-    f = ctypes.c_bool(False)
-    lib.gpu_upload_method_(I(0), ctypes.byref(f), D(1.0))
-    lib.gpu_setup_(I(len(mol.symbols)), I(n), I(10), I(1), I(0), I(2))
-    xyz = np.ascontiguousarray(qb.xyz)      # xyz(3, natom) column-major == [atom][3]
-    lib.gpu_upload_xyz_(xyz.ctypes.data_as(ctypes.c_void_p))
-    MAXPRIM = 14
-    gccoeff = np.zeros((n, MAXPRIM))        # Fortran (MAXPRIM, nbasis) == C [nbasis][MAXPRIM]
-    gcexpo = np.zeros((n, MAXPRIM))
-    for s in range(ns):
-        f0 = qb.ksumtype[s]
-        nf = sum((L + 1) * (L + 2) // 2 for L in range(qb.qstart[s], qb.qfinal[s] + 1))
-        p0, k = qb.kstart[s], qb.kprim[s]
-        for fi in range(f0, f0 + nf):
-            gcexpo[fi, :k] = qb.gcexpo[p0:p0 + k]
-        for L in range(qb.qstart[s], qb.qfinal[s] + 1):
-            gccoeff[f0 + L, :k] = qb.gccoeff[p0:p0 + k, L]
-    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
-    kstart, katom = i32(qb.kstart + 1), i32(qb.katom + 1)
-    kprim, ksum = i32(qb.kprim), i32(qb.ksumtype + 1)
-    qs, qf = i32(qb.qstart), i32(qb.qfinal)
-    dummy_i = np.zeros(max(n, ns) * 4 + 8, dtype=np.int32)
+    n = qb.nbasis
+    vp, D = quick_abi_setup(lib, mol, orc)
     dummy_d = np.zeros(max(n, 1) * 4, dtype=np.float64)
-    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    lib.gpu_upload_basis_.restype = None
-    lib.gpu_upload_basis_(I(ns), I(jb), I(ns), I(jb), I(int(qb.kprim.max())), vp(dummy_i), vp(dummy_i),
-                          vp(dummy_d), vp(dummy_d), vp(dummy_i), vp(dummy_i), vp(dummy_i), vp(dummy_i),
-                          vp(dummy_i), vp(kstart), vp(katom), vp(dummy_i), vp(kprim), vp(dummy_i), vp(ksum),
-                          vp(dummy_i), vp(qs), vp(qf), vp(dummy_i), vp(dummy_i), vp(gccoeff), vp(dummy_d),
-                          vp(gcexpo), vp(dummy_i))
-    Y, CP = orc.schwarz()
-    Yf, CPf = np.asfortranarray(Y), np.asfortranarray(CP)
-    lib.gpu_upload_cutoff_matrix_(Yf.ctypes.data_as(ctypes.c_void_p), CPf.ctypes.data_as(ctypes.c_void_p))
+    f = ctypes.c_bool(False)
+    ierr = ctypes.c_int(0)
     o = np.asfortranarray(np.full((n, n), 0.25))
     dense = np.asfortranarray(P)
     lib.gpu_upload_calculated_(vp(o), vp(dummy_d), vp(dummy_d), dense.ctypes.data_as(ctypes.c_void_p))
