@@ -47,7 +47,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--cpu-builds", type=int, default=5, help="oracle builds timed for the CPU baseline (median)")
-    ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10"],
+    ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10", "uhf"],
                     help="h2o32: (H2O)32 6-31G(d) builds (default); batch: 256 uneven molecules per build "
                          "(plain FP64 vs batched graph FP64 vs batched graph mixed); serve: Poisson SCF "
                          "requests through the continuous-batching server; gly10: mixed-precision SCF error")
@@ -78,6 +78,8 @@ def main():
         return run_serve(a, dev)
     if a.workload == "gly10":
         return run_gly10(a, dev)
+    if a.workload == "uhf":
+        return run_uhf(a, dev)
     if a.copy_probe:
         nbytes = a.copy_probe << 20
         x = torch.ones(nbytes, dtype=torch.uint8, device=dev)
@@ -416,6 +418,46 @@ def run_gly10(a, dev):
         "dE_scf_eh": abs(res["mixed"][0].energy - res["fp64"][0].energy),
         "max_abs_dF_eh": float(np.max(np.abs(Gmx - G64))),
         "scf_iterations": [res["fp64"][0].iterations, res["mixed"][0].iterations],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_uhf(a, dev):
+    """(H2O)32+ doublet UHF/6-31G(d): the open-shell build (two passes of
+    the class kernels: G(Pa+Pb) and the exchange-only G(Pa-Pb)) on the
+    converged UHF densities, against the closed-shell build of the same
+    basis on Pa+Pb."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.autotune import autotune
+    from quick_amd.engine import FockEngine
+    from quick_amd.scf import uhf
+    m0 = M.water_cluster(a.nwater)
+    mol = B.Molecule(m0.symbols, m0.xyz, charge=1)
+    bs = B.build_basis(mol, "6-31G*")
+    e = FockEngine(bs, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff)
+    S, T, V = e.one_electron()
+    r = uhf(e, mult=2, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
+    log(f"[bench] UHF E = {r.energy:.10f} in {r.iterations} it, {r.wall:.2f}s")
+    Pa, Pb = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in r.density)
+    Pt = Pa + Pb
+    sh = torch.cuda.current_stream(dev).cuda_stream
+    if a.autotune:
+        autotune(e, sh, builds=2, make_density=lambda i: Pt.data_ptr())
+    Fa, Fb, F = torch.zeros_like(Pa), torch.zeros_like(Pa), torch.zeros_like(Pa)
+    t_os = _timed(lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh),
+                  a.steps, a.warmup)
+    t_cs = _timed(lambda: e.fock2e_device(Pt.data_ptr(), F.data_ptr(), sh), a.steps, a.warmup)
+    out = {
+        "metric": METRIC, "value": round(1.0 / t_os, 4), "unit": "open-shell Fock builds/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_os * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"(H2O){a.nwater}+ doublet UHF/6-31G(d) open-shell Fock build (Ga, Gb), "
+                               "converged UHF densities", "natom": mol.natom, "nbasis": bs.nbasis,
+                   "integral_cutoff": a.cutoff, "prim_limit": 0.1 * a.cutoff},
+        "closed_shell_ms_same_basis": round(t_cs * 1e3, 4), "open_over_closed": round(t_os / t_cs, 3),
+        "uhf_energy_eh": r.energy, "uhf_iterations": r.iterations, "uhf_wall_s": round(r.wall, 3),
     }
     print(json.dumps(out), flush=True)
 

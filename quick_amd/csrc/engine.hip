@@ -255,6 +255,7 @@ struct qf_engine {
     DevBuf<int> d_colf;
     DevBuf<double> d_Gt, d_Gs, d_cm, d_P;
     DevBuf<double> d_Pb, d_Pt, d_Ps, d_Fa, d_Fb;   // open-shell builds (allocated on first use)
+    DevBuf<double> d_zero;                          // nbasis^2 zeros: Coulomb density of the exchange-only pass
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     qf::MolDesc mol[2]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2
@@ -831,7 +832,7 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
     md.hyb = e->hyb;
     md.replicas = e->replicas;
     md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
-    md.jon = jon;
+    md.PJ = jon ? dP : e->d_zero.p;
     e->mol[slot] = md;
     qf::MolDesc* dmol = e->d_mol.p + slot;
     QF_HIP(hipMemcpyAsync(dmol, &e->mol[slot], sizeof(md), hipMemcpyHostToDevice, s));
@@ -900,6 +901,10 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
     const int n = e->nbasis, ns = e->nshell, n2 = n * n;
     QF_HIP(e->d_Pt.resize(n2));
     QF_HIP(e->d_Ps.resize(n2));
+    if (e->d_zero.n < (size_t)n2) {
+        QF_HIP(e->d_zero.resize(n2));
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, e->d_zero.p, (size_t)n2);
+    }
     hipLaunchKernelGGL(k_spin, dim3((n2 + 255) / 256), dim3(256), 0, s, dPa, dPb, e->d_Pt.p, e->d_Ps.p, n2);
     if (!dcm) {
         const int nthr = 256, tot = ns * ns;
@@ -1470,6 +1475,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         if (d_P && (!d_P[i] || !d_F[i])) return bail(fail(QF_ERR_ARG, "NULL density/Fock pointer"));
         qf::MolDesc m{};
         m.P = d_P ? d_P[i] : b->d_P.p + poff[i];
+        m.PJ = m.P;
         m.G = b->d_Gt.p + gtoff[i];
         m.cutmat = b->d_cm.p + cmoff[i];
         m.nbasis = e->nbasis;
@@ -1477,7 +1483,6 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         m.cutoff = e->cutoff;
         m.primlim = e->primlim;
         m.hyb = e->hyb;
-        m.jon = 1;
         m.replicas = replicas;
         m.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
         md.push_back(m);

@@ -177,8 +177,9 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
     QF_AS_LDS double* Dad = Dac + na * nc;
     QF_AS_LDS double* Dbc = Dad + na * nd;
     QF_AS_LDS double* Dbd = Dbc + nb * nc;
-    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = P[(bi.fa + x / nb) * n + bi.fb + x % nb];
-    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = P[(ki.fa + x / nd) * n + ki.fb + x % nd];
+    const auto PJ = gptr(md.PJ);              // Coulomb density (zeros in an exchange-only build)
+    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = PJ[(bi.fa + x / nb) * n + bi.fb + x % nb];
+    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = PJ[(ki.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < na * nc; x += D::NT) Dac[x] = P[(bi.fa + x / nc) * n + ki.fa + x % nc];
     for (int x = lane; x < na * nd; x += D::NT) Dad[x] = P[(bi.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < nb * nc; x += D::NT) Dbc[x] = P[(bi.fb + x / nc) * n + ki.fa + x % nc];
@@ -190,7 +191,7 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
     constexpr int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
                   o6 = o5 + nb * nd;
 #define IX(a, b, c, d) Ic[(((a) * nb + (b)) * nc + (c)) * nd + (d)]
-    for (int o = (md.jon ? 0 : o2) + lane; o < o6; o += D::NT) {   // jon = 0: exchange blocks only
+    for (int o = lane; o < o6; o += D::NT) {
         double v = 0.0;
         int row, col;
         if (o < o1) {                              // J_ab = sum_cd D_cd I

@@ -52,6 +52,20 @@ __host__ __device__ __forceinline__ double qf_rsqrt(double x)
 #endif
 }
 
+// 1/x: the hardware reciprocal refined by two Newton steps on the device
+// (within an ulp of the division; x = 0 gives NaN, which no caller uses),
+// the division on the host.
+__host__ __device__ __forceinline__ double qf_rcp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 template <typename R>
 struct VrrPar {
     R PA[3], WP[3], QC[3], WQ[3];
@@ -106,7 +120,7 @@ __host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
 template <int L, typename Tab>
 __host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1], Tab tab)
 {
-    const double XINV = 1.0 / X;
+    const double XINV = qf_rcp(X);
     const double E = exp(-X);
     const bool up = (X > 1.0e-1) || (X > 1.0e-4 && maxm < 4);
     const int k = X > 5.0 ? (X > 33.0 ? 6 : X > 15.0 ? 5 : X > 10.0 ? 4 : 3) : (X > 3.0 ? 2 : X > 1.0 ? 1 : 0);
@@ -127,7 +141,7 @@ __host__ __device__ __forceinline__ void boys_quick(const double X, const int ma
     } else {
         // maxm exceeds L only by the s parts of SP shells (at most 4)
         const double twot = X * 2.0;
-        double f = (1.0 - X) / (double)(2 * maxm + 1);
+        double f = (1.0 - X) * qf_rcp(2 * maxm + 1.0);
 #pragma unroll
         for (int m = L + 3; m >= L; --m)
             if (m < maxm) f = (twot * f + E) * (1.0 / (2 * m + 1));
@@ -184,24 +198,31 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
     const auto bpc = cptr(bb.prim);           // wave-uniform bra data
     const auto kp = gptr(kb.prim);            // per-lane ket data
     const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
-#define BP(idx) (BRA_UNIFORM ? bpc[idx] : bp[idx])
-    const double c0 = dn * BP((PF_CUT * bK) * bn + i);     // largest bra cutprim
+    // Field f of primitive q of pair x lives at prim[(f K + q) n + x].  The
+    // loads add a 32-bit byte offset (q n + x) * 8, shared by the fields, to
+    // a wave-uniform field base: one address add per load instead of a
+    // sign-extended 64-bit index computed per lane and field.
+    const size_t bfs = (size_t)bK * bn, kfs = (size_t)kK * kn;
+#define BP(f, q) (BRA_UNIFORM ? *(const QF_AS_CONST double*)((const QF_AS_CONST char*)(bpc + (f) * bfs) + \
+                                                              (uint32_t)((q) * bn + i) * 8u)            \
+                              : bp[(f) * bfs + (size_t)(q) * bn + i])
+#define KP(f, off) (*(const QF_AS_GLOBAL double*)((const QF_AS_GLOBAL char*)(kp + (f) * kfs) + (off)))
+    const double c0 = dn * BP(PF_CUT, 0);     // largest bra cutprim
     for (int jp = 0; jp < kK; ++jp) {
-        const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
+        const uint32_t ko = (uint32_t)(jp * kn + j) * 8u;
+        const double cutk = KP(PF_CUT, ko);
         if (screen && !(c0 * cutk > primlim)) break;
-        const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
-        const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
-        const double Kk = kp[(PF_KC * kK + jp) * kn + j];
-        const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
-                              kp[(PF_PZ * kK + jp) * kn + j]};
+        const double zk = KP(PF_ZETA, ko);
+        const double ozk = KP(PF_OOZ2, ko);
+        const double Kk = KP(PF_KC, ko);
+        const double Qk[3] = {KP(PF_PX, ko), KP(PF_PY, ko), KP(PF_PZ, ko)};
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
         for (int ip = 0; ip < bK; ++ip) {
-            if (screen && !((dn * BP((PF_CUT * bK + ip) * bn + i)) * cutk > primlim)) break;
-            const double zb = BP((PF_ZETA * bK + ip) * bn + i);
-            const double ozb = BP((PF_OOZ2 * bK + ip) * bn + i);
-            const double Pb[3] = {BP((PF_PX * bK + ip) * bn + i), BP((PF_PY * bK + ip) * bn + i),
-                                  BP((PF_PZ * bK + ip) * bn + i)};
-            const double Kb = BP((PF_KC * bK + ip) * bn + i);
+            if (screen && !((dn * BP(PF_CUT, ip)) * cutk > primlim)) break;
+            const double zb = BP(PF_ZETA, ip);
+            const double ozb = BP(PF_OOZ2, ip);
+            const double Pb[3] = {BP(PF_PX, ip), BP(PF_PY, ip), BP(PF_PZ, ip)};
+            const double Kb = BP(PF_KC, ip);
             const double rs = qf_rsqrt(zb + zk);       // 1/sqrt(zeta+eta)
             const double inv = rs * rs;
             const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
@@ -231,6 +252,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
         }
     }
 #undef BP
+#undef KP
 }
 
 // ---------------------------------------------------------------------------
@@ -243,8 +265,8 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 //   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc I, Gt_bc -= .. D_ad I, Gt_bd -= .. D_ac I,
 // and G = Gt + Gt^T afterwards.  Because a is fixed for the wave, J_cd, K_ac
 // and K_ad stay in registers across the bra loop and J_ab is summed over
-// the wave per bra.  With md.jon = 0 the Coulomb terms are skipped
-// (the exchange-only pass of the open-shell build).
+// the wave per bra.  The Coulomb terms read their density from
+// md.PJ: P itself, or zeros in the exchange-only pass of the open-shell build.
 //
 // TILE = false: everything goes straight to G with FP64 atomics (K_bc and
 // K_bd per quartet).  TILE = true: J_ab and every exchange term go to the
@@ -274,7 +296,8 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     const int n = md.nbasis, ns = md.nshell;
     const auto cmc = cptr(md.cutmat);
     const auto cm = gptr(md.cutmat);
-    const auto Pc = cptr(md.P);
+    const auto PJc = cptr(md.PJ);
+    const auto PJ = gptr(md.PJ);
     const auto P = gptr(md.P);
     const auto bperm = cptr(bb.bperm);
 
@@ -289,7 +312,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = P[(ki.fa + c) * n + ki.fb + d];
+        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = PJ[(ki.fa + c) * n + ki.fb + d];
 #pragma unroll
     for (int a = 0; a < na; ++a) {
 #pragma unroll
@@ -338,7 +361,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
-            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = Pc[(bi.fa + a) * n + bi.fb + b];
+            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = PJc[(bi.fa + a) * n + bi.fb + b];
 #pragma unroll
         for (int b = 0; b < nb; ++b) {
 #pragma unroll
@@ -371,20 +394,18 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         Kbd[b * nd + d] += Dac[a * nc + c] * v;
                     }
         // bra block: summed over the wave
-        if (md.jon) {
 #pragma unroll
-            for (int a = 0; a < na; ++a)
+        for (int a = 0; a < na; ++a)
 #pragma unroll
-                for (int b = 0; b < nb; ++b) {
-                    const double v = wave_sum(Jab[a * nb + b]);
-                    if (lane == 0 && v != 0.0) {
-                        if (TILE)
-                            lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
-                        else
-                            atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
-                    }
+            for (int b = 0; b < nb; ++b) {
+                const double v = wave_sum(Jab[a * nb + b]);
+                if (lane == 0 && v != 0.0) {
+                    if (TILE)
+                        lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
+                    else
+                        atomic_add(&G[(bi.fa + a) * n + bi.fb + b], v);
                 }
-        }
+            }
         if (valid) {
             if (TILE) {
                 const int r0 = na + (kk - tr.chunk_kb) * nb;
@@ -410,12 +431,10 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         }
     }
     if (!touched) return;
-    if (md.jon) {
 #pragma unroll
-        for (int c = 0; c < nc; ++c)
+    for (int c = 0; c < nc; ++c)
 #pragma unroll
-            for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
-    }
+        for (int d = 0; d < nd; ++d) atomic_add(&G[(ki.fa + c) * n + ki.fb + d], Jcd[c * nd + d]);
     if (TILE) {
 #pragma unroll
         for (int a = 0; a < na; ++a) {

@@ -38,13 +38,14 @@ struct PairBlock {
 
 struct MolDesc {
     const double* P;        // nbasis x nbasis density, row-major
+    const double* PJ;       // density of the Coulomb terms: P, or zeros for an exchange-only build
     double* G;              // nbasis x nbasis accumulator (G = Gt + Gt^T afterwards)
     const double* cutmat;   // nshell x nshell max |P| per original shell pair
     int nbasis, nshell;
     double cutoff, primlim, hyb;
     double split;           // quartets with Ycut*Ycut >= split run in FP64, below in FP32
     int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
-    int jon;                // 1: Coulomb and exchange terms; 0: exchange terms only
+    int pad;
 };
 
 // One wave: bras bperm[kb..ke) of one bra block - all with the same first
