@@ -406,10 +406,13 @@ def quartet_classes():
 
 def min_waves(nacc, ni):
     """Occupancy target per class (caps VGPRs: 4 waves -> 128, 3 -> 168, 2 -> 256).
-    Even (ss|ss) needs ~150 VGPRs for the Boys function and the digestion
-    state, so no class is capped below 168 (at 128 the spills cost more than
-    the fourth wave gains)."""
+    (ss|ss) fits 128 VGPRs without spilling and gets 4 waves; every other
+    class needs more than 128 for the Boys function and the digestion state
+    (at 128 its spills cost more than the fourth wave gains), so it is
+    capped at 168."""
     size = nacc + ni
+    if size <= 2:
+        return 4
     if size <= 40:
         return 3
     if size <= 80:
