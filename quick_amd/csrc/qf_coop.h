@@ -85,20 +85,22 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
 #pragma unroll
             for (int m = 0; m <= D::L; ++m)
                 if (lane == m) s[D::BASE + m] = F[m] * pref;
-            if (lane < 24) {
-                const int sel = lane >> 2, k = lane & 3, dir = sel % 3;
-                const double pa = dir == 0 ? Pb[0] - bi.A[0] : dir == 1 ? Pb[1] - bi.A[1] : Pb[2] - bi.A[2];
-                const double pq = dir == 0 ? PQ[0] : dir == 1 ? PQ[1] : PQ[2];
-                const double qc = dir == 0 ? QC[0] : dir == 1 ? QC[1] : QC[2];
-                double v;
-                if (sel < 3)
-                    v = k == 0 ? pa : k == 1 ? -wp * pq : k == 2 ? ozb : wp;
-                else
-                    v = k == 0 ? qc : k == 1 ? wq * pq : k == 2 ? ozk : wq;
-                cf[lane] = v;
-            } else if (lane == 24) {
+            if (lane == 0) {
+                // the values are the same for the whole workgroup (one quartet):
+                // one lane stores them with compile-time indices (a lane-indexed
+                // read would place the pair records and these arrays in scratch)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    cf[4 * d + 0] = Pb[d] - bi.A[d];
+                    cf[4 * d + 1] = -wp * PQ[d];
+                    cf[4 * d + 2] = ozb;
+                    cf[4 * d + 3] = wp;
+                    cf[12 + 4 * d + 0] = QC[d];
+                    cf[12 + 4 * d + 1] = wq * PQ[d];
+                    cf[12 + 4 * d + 2] = ozk;
+                    cf[12 + 4 * d + 3] = wq;
+                }
                 cf[24] = 0.5 * inv;
-            } else if (lane == 25) {
                 s[D::ZS] = 0.0;
             }
             __syncthreads();
@@ -134,9 +136,12 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
     }
 
     // HRR on the contracted block (the VRR region is dead now)
-    if (lane < 3) {
-        cf[25 + lane] = ki.AB[lane];
-        cf[28 + lane] = bi.AB[lane];
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            cf[25 + d] = ki.AB[d];
+            cf[28 + d] = bi.AB[d];
+        }
     }
     __syncthreads();
     const auto hrr = gptr(D::hrr());
