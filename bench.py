@@ -47,7 +47,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--cpu-builds", type=int, default=5, help="oracle builds timed for the CPU baseline (median)")
-    ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10", "uhf"],
+    ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10", "uhf", "quick_tests"],
                     help="h2o32: (H2O)32 6-31G(d) builds (default); batch: 256 uneven molecules per build "
                          "(plain FP64 vs batched graph FP64 vs batched graph mixed); serve: Poisson SCF "
                          "requests through the continuous-batching server; gly10: mixed-precision SCF error")
@@ -80,6 +80,8 @@ def main():
         return run_gly10(a, dev)
     if a.workload == "uhf":
         return run_uhf(a, dev)
+    if a.workload == "quick_tests":
+        return run_quick_tests(a, dev)
     if a.copy_probe:
         nbytes = a.copy_probe << 20
         x = torch.ones(nbytes, dtype=torch.uint8, device=dev)
@@ -458,6 +460,55 @@ def run_uhf(a, dev):
                    "integral_cutoff": a.cutoff, "prim_limit": 0.1 * a.cutoff},
         "closed_shell_ms_same_basis": round(t_cs * 1e3, 4), "open_over_closed": round(t_os / t_cs, 3),
         "uhf_energy_eh": r.energy, "uhf_iterations": r.iterations, "uhf_wall_s": round(r.wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_quick_tests(a, dev):
+    """QUICK's own RHF regression inputs (test/*.in, geometries and saved
+    outputs in tests/golden/saved_rhf_energies.json): the SCF through the
+    host build path (qf_fock_build_host: upload of P, build, download of G,
+    as gpu_get_cshell_eri_ does), at each input's cutoff.  Reports the
+    median time per Fock build against the saved output's TOTAL 2e TIME per
+    SCF cycle (QUICK on an NVIDIA TITAN V, also host wall time around the
+    build including its transfers), and the energy against the saved one."""
+    from quick_amd import basis as B
+    from quick_amd.engine import FockEngine
+    from quick_amd.scf import rhf
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "tests", "golden", "saved_rhf_energies.json")) as fh:
+        recs = json.load(fh)["records"]
+    rows = []
+    for r in recs:
+        if "t2e_total_s" not in r or not r.get("cycles"):
+            continue
+        mol = B.Molecule.from_angstrom(r["symbols"], r["xyz_angstrom"], charge=r["charge"])
+        try:
+            bs = B.build_basis(mol, r["basis"])
+        except KeyError:
+            continue
+        if bs.nbasis != r["nbasis"]:
+            continue
+        e = FockEngine(bs, cutoff=r["cutoff"], prim_limit=0.1 * r["cutoff"])
+        res = rhf(e, etol=1e-8, dtol=1e-6)
+        e.close()
+        ours = 1e3 * float(np.median(res.fock_times[1:]))
+        quick = 1e3 * r["t2e_total_s"] / r["cycles"]
+        rows.append({"name": r["name"], "nbasis": bs.nbasis, "ms_per_build": round(ours, 4),
+                     "quick_ms_per_build": round(quick, 4), "speedup": round(quick / ours, 2),
+                     "dE_vs_saved_eh": float(abs(res.energy - r["E_total"])), "converged": bool(res.converged)})
+        log(f"[bench] {r['name']:28s} n={bs.nbasis:4d} {ours:8.3f} ms/build  QUICK {quick:8.3f}  "
+            f"x{quick / ours:6.1f}  dE {abs(res.energy - r['E_total']):.1e}")
+    sp = np.array([x["speedup"] for x in rows])
+    big = [x for x in rows if x["nbasis"] >= 100]
+    out = {
+        "metric": "Fock build time vs QUICK's saved TITAN V timings", "value": round(float(np.exp(np.mean(np.log(sp)))), 3),
+        "unit": "geometric-mean speedup over QUICK (TITAN V)", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64", "data": "QUICK test inputs",
+        "config": {"workload": f"{len(rows)} RHF regression inputs of QUICK, host build path incl. transfers",
+                   "reference_device": "NVIDIA TITAN V (saved outputs)"},
+        "speedup_nbasis_ge_100": [{"name": x["name"], "speedup": x["speedup"]} for x in big],
+        "max_dE_vs_saved_eh": max(x["dE_vs_saved_eh"] for x in rows), "rows": rows,
     }
     print(json.dumps(out), flush=True)
 

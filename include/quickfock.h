@@ -200,6 +200,12 @@ int qf_set_class_tuning(qf_engine *e, int cb, int ck, int tile, int waves, int t
 /* Persistent launches: at most max_workgroups workgroups per class launch,
  * each striding over the tasks (0: one workgroup per task / tile group). */
 int qf_set_persistent(qf_engine *e, int max_workgroups);
+/* Primitive pieces (no QUICK counterpart: a launch-shape choice).  Each task
+ * sums at most `piece` bra and `piece` ket primitive pairs of its quartets, so
+ * the most contracted quartets of a small molecule are spread over several
+ * waves; the pieces add up to the same Fock matrix.  piece = 0: automatic (6
+ * up to 256 basis functions, no split above); piece < 0: never split. */
+int qf_set_prim_split(qf_engine *e, int piece);
 
 /* HIP-event timing of every class launch (on the build stream); read sums
  * the durations per launch index since the last read. */

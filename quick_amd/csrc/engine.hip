@@ -189,10 +189,10 @@ __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf
         if (!(y > md.cutoff && y * dn > md.cutoff)) continue;
         if (t.same_block && j > i) continue;
         if (fp32 ? !(y < md.split) : !(t.all_fp64 || y >= md.split)) continue;
-        ++nq;
-        for (int ip = 0; ip < bb.K; ++ip) {
+        if (t.pb == 0 && t.qb == 0) ++nq;   // a quartet spread over several tasks counts once
+        for (int ip = t.pb; ip < t.pe; ++ip) {
             const double c1 = dn * bb.prim[(qf::PF_CUT * bb.K + ip) * bb.n + i];
-            for (int jp = 0; jp < kb.K; ++jp)
+            for (int jp = t.qb; jp < t.qe; ++jp)
                 if (c1 * kb.prim[(qf::PF_CUT * kb.K + jp) * kb.n + j] > md.primlim) ++np;
         }
     }
@@ -226,6 +226,7 @@ struct qf_engine {
     int tile_tasks = 16;     // max tasks per tile workgroup
     int lds_budget = 64 * 1024;
     int persist = 0;         // grid cap of the class launches (0: one workgroup per task / group)
+    int prim_piece = 0;      // primitive pairs per task piece: 0 auto, < 0 never split
     struct ClassTune {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
     };
@@ -616,6 +617,11 @@ int prepare(qf_engine* e)
     e->launches.clear();
     e->stats = qf_stats{};
     for (auto& b : e->blocks) e->stats.n_pairs += (long long)b.info.size();
+    // Primitive pieces: a wave's lanes each run the whole primitive loop of
+    // one quartet, so for small molecules (few waves) the most contracted
+    // quartets set the time of a launch.  Below 256 functions the primitive
+    // pairs of a task are cut into pieces of at most 6 (e.g. 36 -> 6 x 6).
+    const int piece = e->prim_piece > 0 ? e->prim_piece : (e->prim_piece == 0 && e->nbasis <= 256) ? 6 : 1 << 20;
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32) {
@@ -700,7 +706,18 @@ int prepare(qf_engine* e)
                                 t.ket_count = std::min(win, nmax - s0);
                                 t.same_block = same ? 1 : 0;
                                 t.all_fp64 = split_class ? 0 : 1;
-                                e->tasks.push_back(t);
+                                // highly contracted pairs: spread the primitive
+                                // pairs of the window over several tasks
+                                const int nbp = (B.K + piece - 1) / piece, nkp = (Kb.K + piece - 1) / piece;
+                                const int sb = (B.K + nbp - 1) / nbp, sk = (Kb.K + nkp - 1) / nkp;
+                                for (int pbi = 0; pbi < B.K; pbi += sb)
+                                    for (int qbi = 0; qbi < Kb.K; qbi += sk) {
+                                        t.pb = pbi;
+                                        t.pe = std::min(B.K, pbi + sb);
+                                        t.qb = qbi;
+                                        t.qe = std::min(Kb.K, qbi + sk);
+                                        e->tasks.push_back(t);
+                                    }
                             }
                             if (!tile) continue;
                             // workgroups of the chunk: consecutive runs of its tasks
@@ -722,7 +739,12 @@ int prepare(qf_engine* e)
                 }
                 const int count = (int)e->tasks.size() - offset;
                 const int gcount = (int)e->groups.size() - goffset;
-                auto cost = [](const qf::Task& x) { return (long long)(x.ke - x.kb) * x.ket_count; };
+                // work of a task: bras x kets x primitive pairs (bra and ket);
+                // without the primitive factor the longest-first order misjudges
+                // the contracted classes (13 % of the (H2O)32 build)
+                auto cost = [](const qf::Task& x) {
+                    return (long long)(x.ke - x.kb) * x.ket_count * (x.pe - x.pb) * (x.qe - x.qb);
+                };
                 if (tile && gcount > 0) {
                     // longest workgroups first so the launch ends evenly
                     std::vector<long long> gc(gcount, 0);
@@ -1240,6 +1262,14 @@ int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int t
     return QF_OK;
 }
 
+int qf_set_prim_split(qf_engine* e, int piece)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    e->prim_piece = piece;
+    e->prepared = false;
+    return QF_OK;
+}
+
 int qf_set_persistent(qf_engine* e, int max_workgroups)
 {
     if (!e || max_workgroups < 0) return fail(QF_ERR_ARG, "max_workgroups must be >= 0");
@@ -1500,7 +1530,10 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
     }
     std::vector<qf::Task> tasks;
     std::vector<qf::TileGroup> groups;
-    auto cost = [](const qf::Task& x) { return (long long)(x.ke - x.kb) * x.ket_count; };
+    // work of a task: bras x kets x primitive pairs (bra and ket)
+    auto cost = [](const qf::Task& x) {
+        return (long long)(x.ke - x.kb) * x.ket_count * (x.pe - x.pb) * (x.qe - x.qb);
+    };
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32)

@@ -37,9 +37,9 @@ constexpr int kHU = 4;   // HRR nodes per thread in flight
 
 
 template <class D>
-__device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
-                                             int j, const PairInfo& ki, double dn, const MolDesc& md,
-                                             QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
+__device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb, int i, const PairInfo& bi,
+                                             const PairBlock& kb, int j, const PairInfo& ki, double dn,
+                                             const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
                                              const QF_AS_LDS double* boys_tab)
 {
     constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
@@ -57,7 +57,7 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
     const auto vrr = gptr(D::vrr());
     const auto vlev = cptr(D::vlev());
     const auto tgt = gptr(D::tgt());
-    for (int jp = 0; jp < kK; ++jp) {
+    for (int jp = t.qb; jp < t.qe; ++jp) {
         const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
         if (!(c0 * cutk > md.primlim)) break;
         const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
@@ -66,7 +66,7 @@ __device__ __forceinline__ void coop_quartet(const PairBlock& bb, int i, const P
                               kp[(PF_PZ * kK + jp) * kn + j]};
         const double Kk = kp[(PF_KC * kK + jp) * kn + j];
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
-        for (int ip = 0; ip < bK; ++ip) {
+        for (int ip = t.pb; ip < t.pe; ++ip) {
             if (!((dn * bp[(PF_CUT * bK + ip) * bn + i]) * cutk > md.primlim)) break;
             const double zb = bp[(PF_ZETA * bK + ip) * bn + i];
             const double ozb = bp[(PF_OOZ2 * bK + ip) * bn + i];
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
                 const int j = t.ket_start + v;
                 const PairInfo ki = cptr(kb.info)[j];
                 const double dnv = __shfl(dn, v, WAVE);
-                coop_quartet<D>(bb, i, bi, kb, j, ki, dnv, md, G, s, QF_BOYS_LDS_PTR);
+                coop_quartet<D>(t, bb, i, bi, kb, j, ki, dnv, md, G, s, QF_BOYS_LDS_PTR);
             }
         }
     }

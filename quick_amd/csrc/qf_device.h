@@ -182,7 +182,10 @@ __device__ __forceinline__ void lds_add(QF_AS_LDS double* p, double v)
 // QUICK's primitive screen (dnmax*cutprim(ij))*cutprim(kl) > primLimit
 // (eri.fh:270,288-289).  Primitive pairs are sorted by cutprim descending,
 // so a failing bra primitive ends the inner loop and a failing first bra
-// primitive ends the outer loop: the same set QUICK skips.
+// primitive ends the outer loop: the same set QUICK skips.  [pb, pe) and
+// [qb, qe) restrict the bra and ket primitive pairs (a piece of a quartet
+// spread over several tasks); the screen and its breaks are unchanged, as
+// the first bra primitive still sets the outer test.
 //
 // The ket primitive (different per lane) is the outer loop: its seven
 // fields are vector-loaded once per ket primitive (prefetching the next one
@@ -192,7 +195,8 @@ __device__ __forceinline__ void lds_add(QF_AS_LDS double* p, double v)
 template <class C, typename R, bool BRA_UNIFORM, typename Tab>
 __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
                                          int j, const PairInfo& ki, double dn, double primlim, bool screen,
-                                         int maxm, R (&acc)[C::NACC], Tab boys_tab)
+                                         int maxm, R (&acc)[C::NACC], Tab boys_tab, int pb, int pe, int qb,
+                                         int qe)
 {
     const auto bp = BRA_UNIFORM ? (const double*)nullptr : bb.prim;
     const auto bpc = cptr(bb.prim);           // wave-uniform bra data
@@ -208,7 +212,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
                               : bp[(f) * bfs + (size_t)(q) * bn + i])
 #define KP(f, off) (*(const QF_AS_GLOBAL double*)((const QF_AS_GLOBAL char*)(kp + (f) * kfs) + (off)))
     const double c0 = dn * BP(PF_CUT, 0);     // largest bra cutprim
-    for (int jp = 0; jp < kK; ++jp) {
+    for (int jp = qb; jp < qe; ++jp) {
         const uint32_t ko = (uint32_t)(jp * kn + j) * 8u;
         const double cutk = KP(PF_CUT, ko);
         if (screen && !(c0 * cutk > primlim)) break;
@@ -217,7 +221,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
         const double Kk = KP(PF_KC, ko);
         const double Qk[3] = {KP(PF_PX, ko), KP(PF_PY, ko), KP(PF_PZ, ko)};
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
-        for (int ip = 0; ip < bK; ++ip) {
+        for (int ip = pb; ip < pe; ++ip) {
             if (screen && !((dn * BP(PF_CUT, ip)) * cutk > primlim)) break;
             const double zb = BP(PF_ZETA, ip);
             const double ozb = BP(PF_OOZ2, ip);
@@ -347,7 +351,9 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         R acc[C::NACC];
 #pragma unroll
         for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
-        if (valid) contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc, tr.boys);
+        if (valid)
+            contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc, tr.boys, t.pb, t.pe,
+                                 t.qb, t.qe);
         R I[C::NI];
         {
             const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
@@ -561,7 +567,8 @@ __global__ __launch_bounds__(WAVE) void schwarz_kernel(PairBlock blk, double* __
     double acc[C::NACC];
 #pragma unroll
     for (int k = 0; k < C::NACC; ++k) acc[k] = 0.0;
-    contract<C, double, false>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc, QF_BOYS_LDS_PTR);
+    contract<C, double, false>(blk, j, pi, blk, j, pi, 0.0, 0.0, false, 2 * pi.qfs, acc, QF_BOYS_LDS_PTR, 0, blk.K, 0,
+                               blk.K);
     double I[C::NI];
     const double AB[3] = {pi.AB[0], pi.AB[1], pi.AB[2]};
     C::template hrr<double>(acc, AB, AB, I);

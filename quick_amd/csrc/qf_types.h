@@ -54,6 +54,11 @@ struct MolDesc {
 struct Task {
     int bra_block, kb, ke, ket_block, ket_start, ket_count, same_block;
     int all_fp64;   // 1: this class runs every quartet in FP64 even in mixed mode
+    // primitive-pair ranges of the bra and ket pairs this task sums over
+    // ([0, K) unless a highly contracted class is spread over several tasks:
+    // the HRR and the digestion are linear in the primitive sums, so the
+    // pieces add up to the full quartet)
+    int pb, pe, qb, qe;
 };
 
 // One workgroup of the LDS-tile kernels: tasks [task_begin, task_end), all

@@ -181,6 +181,10 @@ class FockEngine:
     def set_persistent(self, max_workgroups=0):
         L.check(self.lib.qf_set_persistent(self.h, int(max_workgroups)), "qf_set_persistent")
 
+    def set_prim_split(self, piece=0):
+        """Primitive pairs per task piece: 0 automatic, < 0 never split."""
+        L.check(self.lib.qf_set_prim_split(self.h, int(piece)), "qf_set_prim_split")
+
     def set_timing(self, on=True):
         L.check(self.lib.qf_set_timing(self.h, 1 if on else 0), "qf_set_timing")
 

@@ -2,7 +2,8 @@
 
 Reads the reference's ``test/ene_*_rhf_*.in`` inputs (geometry, basis,
 charge) and the matching ``test/saved/*.out`` outputs (TOTAL ENERGY,
-ELECTRONIC ENERGY, CORE_CORE REPULSION, and the SCF cycle count) and writes
+ELECTRONIC ENERGY, CORE_CORE REPULSION, the SCF cycle count, and the
+TOTAL 2e TIME of the Fock builds with the GPU it ran on) and writes
 ``saved_rhf_energies.json`` beside this script.  The ``*_uhf_*.in`` inputs
 (with their multiplicity) go to ``saved_uhf_energies.json``; for a geometry
 optimisation the record is the first SCF, at the input geometry.  Only data
@@ -55,6 +56,12 @@ def parse_out(path):
         m = re.search(tag + r"\s*=\s*(-?\d+\.\d+)", txt)
         if m:
             out[key] = float(m.group(1))
+    m = re.search(r"TOTAL 2e TIME\s*=\s*(\d+\.\d+)", txt)
+    if m:
+        out["t2e_total_s"] = float(m.group(1))      # host wall time of all Fock builds of the run
+    m = re.search(r"(?:CUDA|HIP) DEVICE NAME\s*:\s*(.+?)\s*$", txt, re.M)
+    if m:
+        out["device"] = m.group(1)
     m = re.search(r"REACH CONVERGENCE AFTER\s+(\d+)", txt)
     if m:
         out["cycles"] = int(m.group(1))
