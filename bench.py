@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--autotune", type=int, default=1, help="tune launch shapes per class before timing")
+    ap.add_argument("--prim-split", type=int, default=0,
+                    help="primitive pairs per task piece (0: automatic, < 0: never split)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -98,6 +100,8 @@ def main():
     t0 = time.perf_counter()
     eng_replicas = 8
     eng = FockEngine(bs, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, precision=a.precision)
+    if a.prim_split:
+        eng.set_prim_split(a.prim_split)
     eng.prepare()
     t_setup = time.perf_counter() - t0
     st = eng.stats()

@@ -204,7 +204,7 @@ int qf_set_persistent(qf_engine *e, int max_workgroups);
  * sums at most `piece` bra and `piece` ket primitive pairs of its quartets, so
  * the most contracted quartets of a small molecule are spread over several
  * waves; the pieces add up to the same Fock matrix.  piece = 0: automatic (6
- * up to 256 basis functions, no split above); piece < 0: never split. */
+ * up to 256 basis functions, 12 above); piece < 0: never split. */
 int qf_set_prim_split(qf_engine *e, int piece);
 
 /* HIP-event timing of every class launch (on the build stream); read sums
