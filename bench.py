@@ -494,6 +494,8 @@ def run_quick_tests(a, dev):
         if bs.nbasis != r["nbasis"]:
             continue
         e = FockEngine(bs, cutoff=r["cutoff"], prim_limit=0.1 * r["cutoff"])
+        if a.prim_split:
+            e.set_prim_split(a.prim_split)
         res = rhf(e, etol=1e-8, dtol=1e-6)
         e.close()
         ours = 1e3 * float(np.median(res.fock_times[1:]))

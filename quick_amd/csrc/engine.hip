@@ -620,11 +620,16 @@ int prepare(qf_engine* e)
     // Primitive pieces: a wave's lanes each run the whole primitive loop of
     // one quartet, so the most contracted quartets set the tail of a launch
     // (and, for small molecules with few waves, its whole time).  The
-    // primitive pairs of a task are cut into pieces of at most 6 up to 256
-    // functions and 12 above (e.g. 36 -> 6 x 6 or 3 x 12).  Measured on
+    // primitive pairs of a task are cut into pieces of at most 2 up to 128
+    // functions, 4 up to 256 and 12 above (e.g. 36 -> 3 x 12).  Measured on
     // (H2O)32 6-31G(d): no split 17.0 ms, pieces of 6 / 12 / 18: 15.8 / 15.4 /
-    // 15.9 ms.
-    const int piece = e->prim_piece > 0 ? e->prim_piece : e->prim_piece < 0 ? 1 << 20 : e->nbasis <= 256 ? 6 : 12;
+    // 15.9 ms; on QUICK's regression inputs (13-210 functions) pieces of 2-4
+    // beat 6.
+    const int piece = e->prim_piece > 0   ? e->prim_piece
+                      : e->prim_piece < 0 ? 1 << 20
+                      : e->nbasis <= 128  ? 2
+                      : e->nbasis <= 256  ? 4
+                                          : 12;
     for (int cb = 0; cb < 6; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32) {
@@ -656,7 +661,15 @@ int prepare(qf_engine* e)
                                          kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
-                const bool coop = !fp32 && ct.tile == 2 && qf::fock_coop_launcher(cb, ck) != nullptr;
+                // wave-per-quartet kernel: where the class tuning asks for it, and by
+                // default (unless qf_set_tile turned the shared-memory kernels off)
+                // for the d-heavy classes (La+Lb+Lc+Ld >= 6), whose per-lane
+                // kernels spill several KB per lane: one such quartet alone on a
+                // lane costs milliseconds, which dominates the build of a small
+                // molecule
+                const int Lsum = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
+                const bool want_coop = ct.tile == 2 || (ct.tile < 0 && e->tile != 0 && Lsum >= 6);
+                const bool coop = !fp32 && want_coop && qf::fock_coop_launcher(cb, ck) != nullptr;
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
                 const int win = coop ? kCoopWindow : qf::WAVE;
