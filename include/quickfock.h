@@ -186,7 +186,8 @@ int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
  * per quartet.  waves: waves per workgroup (1..4); tasks_per_group: tasks
  * one workgroup takes; lds_bytes: tile budget (1 KiB..64 KiB).  Classes
  * whose tile does not fit fall back to direct atomics.  With enable = 1 the
- * d-heavy classes (La+Lb+Lc+Ld >= 6) default to the wave-per-quartet kernel;
+ * d-heavy classes (La+Lb+Lc+Ld >= 6, or >= 4 up to 128 basis functions)
+ * default to the wave-per-quartet kernel;
  * enable = 0 runs every class without a qf_set_class_tuning entry on the
  * per-lane direct-atomic kernels. */
 int qf_set_tile(qf_engine *e, int enable, int waves, int tasks_per_group, int lds_bytes);

@@ -663,12 +663,14 @@ int prepare(qf_engine* e)
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
                 // default (unless qf_set_tile turned the shared-memory kernels off)
-                // for the d-heavy classes (La+Lb+Lc+Ld >= 6), whose per-lane
+                // for the d-heavy classes (La+Lb+Lc+Ld >= 6; >= 4 up to 128 functions,
+                // where few waves run and latency decides), whose per-lane
                 // kernels spill several KB per lane: one such quartet alone on a
                 // lane costs milliseconds, which dominates the build of a small
                 // molecule
                 const int Lsum = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
-                const bool want_coop = ct.tile == 2 || (ct.tile < 0 && e->tile != 0 && Lsum >= 6);
+                const bool want_coop =
+                    ct.tile == 2 || (ct.tile < 0 && e->tile != 0 && Lsum >= (e->nbasis <= 128 ? 4 : 6));
                 const bool coop = !fp32 && want_coop && qf::fock_coop_launcher(cb, ck) != nullptr;
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
