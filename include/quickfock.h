@@ -210,6 +210,9 @@ int qf_set_persistent(qf_engine *e, int max_workgroups);
  * waves; the pieces add up to the same Fock matrix.  piece = 0: automatic (6
  * up to 256 basis functions, 12 above); piece < 0: never split. */
 int qf_set_prim_split(qf_engine *e, int piece);
+/* Per-class primitive piece size (class indices as qf_set_class_tuning):
+ * 0 follows qf_set_prim_split, < 0 never splits the class. */
+int qf_set_class_piece(qf_engine *e, int cb, int ck, int piece);
 
 /* HIP-event timing of every class launch (on the build stream); read sums
  * the durations per launch index since the last read. */

@@ -229,6 +229,7 @@ struct qf_engine {
     int prim_piece = 0;      // primitive pairs per task piece: 0 auto, < 0 never split
     struct ClassTune {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
+        int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
     };
     ClassTune ctune[6][6];
     // pair blocks
@@ -726,7 +727,8 @@ int prepare(qf_engine* e)
                                 t.all_fp64 = split_class ? 0 : 1;
                                 // highly contracted pairs: spread the primitive
                                 // pairs of the window over several tasks
-                                const int nbp = (B.K + piece - 1) / piece, nkp = (Kb.K + piece - 1) / piece;
+                                const int cpiece = ct.piece > 0 ? ct.piece : ct.piece < 0 ? 1 << 20 : piece;
+                                const int nbp = (B.K + cpiece - 1) / cpiece, nkp = (Kb.K + cpiece - 1) / cpiece;
                                 const int sb = (B.K + nbp - 1) / nbp, sk = (Kb.K + nkp - 1) / nkp;
                                 for (int pbi = 0; pbi < B.K; pbi += sb)
                                     for (int qbi = 0; qbi < Kb.K; qbi += sk) {
@@ -1276,6 +1278,15 @@ int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int t
     t.waves = std::max(0, waves);
     t.tpg = std::max(0, tasks_per_group);
     t.mmax = std::max(0, bras_per_task);
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_class_piece(qf_engine* e, int cb, int ck, int piece)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    e->ctune[cb][ck].piece = piece;
     e->prepared = false;
     return QF_OK;
 }

@@ -185,6 +185,10 @@ class FockEngine:
         """Primitive pairs per task piece: 0 automatic, < 0 never split."""
         L.check(self.lib.qf_set_prim_split(self.h, int(piece)), "qf_set_prim_split")
 
+    def set_class_piece(self, cb, ck, piece=0):
+        """Primitive piece size of one class: 0 engine default, < 0 never split."""
+        L.check(self.lib.qf_set_class_piece(self.h, int(cb), int(ck), int(piece)), "qf_set_class_piece")
+
     def set_timing(self, on=True):
         L.check(self.lib.qf_set_timing(self.h, 1 if on else 0), "qf_set_timing")
 
