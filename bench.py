@@ -11,8 +11,13 @@ F += G).
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|mixed]
 
 Prints one JSON line.  The CPU baseline is the oracle (the C restatement of
-QUICK's CPU get2e) running one full build of the same workload with OpenMP on
-the host cores.
+QUICK's CPU get2e) running full builds of the same workload with OpenMP on
+the host cores (median of --cpu-builds), plus a bounded single-thread sample.
+
+Other workloads (--workload): batch (256 uneven molecules per build), serve
+(continuous-batching SCF server), gly10 (split precision), uhf (open-shell
+build) and quick_tests (QUICK's own RHF regression inputs against the build
+times in their saved outputs).
 """
 from __future__ import annotations
 
@@ -50,7 +55,9 @@ def main():
     ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10", "uhf", "quick_tests"],
                     help="h2o32: (H2O)32 6-31G(d) builds (default); batch: 256 uneven molecules per build "
                          "(plain FP64 vs batched graph FP64 vs batched graph mixed); serve: Poisson SCF "
-                         "requests through the continuous-batching server; gly10: mixed-precision SCF error")
+                         "requests through the continuous-batching server; gly10: mixed-precision SCF error; "
+                         "uhf: open-shell build of (H2O)32+; quick_tests: QUICK's 69 RHF regression inputs "
+                         "against the build times of their saved outputs")
     ap.add_argument("--molecules", type=int, default=256)
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
