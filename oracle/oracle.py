@@ -310,8 +310,9 @@ def seeded_density(S, nocc, seed=20261202):
 
 
 def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=1e-10,
-        dtol=1e-8, ndiis=10, verbose=False):
-    """Plain-DIIS RHF with full (non-incremental) Fock builds by the oracle."""
+        dtol=1e-8, ndiis=10, verbose=False, threads=1):
+    """Plain-DIIS RHF with full (non-incremental) Fock builds by the oracle
+    (``threads`` > 1: the OpenMP build, equal to the serial one)."""
     qb = oracle.qb
     S, T, V = oracle.one_electron()
     H = T + V
@@ -330,7 +331,7 @@ def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=
     fs, es = [], []
     hist = []
     for it in range(maxiter):
-        G, _ = oracle.fock2e(P, cutoff=cutoff, prim_limit=prim_limit)
+        G, _ = oracle.fock2e(P, cutoff=cutoff, prim_limit=prim_limit, threads=threads)
         F = H + G
         E = 0.5 * np.sum(P * (H + F)) + enuc
         err = X.T @ (F @ P @ S - S @ P @ F) @ X

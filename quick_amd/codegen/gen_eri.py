@@ -257,6 +257,18 @@ COOP_CLASSES = [(1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 1), (2, 1, 2
                 (2, 2, 1, 0), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
 
 
+# Classes with La+Lb+Lc+Ld above this get no per-lane kernels: their per-lane
+# forms need several KB of scratch per lane (a (dd|dd) quartet holds 961
+# contracted accumulators), and provisioning that private segment for a
+# full-occupancy launch is what the runtime cannot always do.  Every launch of
+# such a class runs on the wave-per-quartet kernel.
+LANE_LSUM_MAX = 5
+
+
+def has_lane_kernels(La, Lb, Lc, Ld):
+    return La + Lb + Lc + Ld <= LANE_LSUM_MAX
+
+
 def coop_tables(La, Lb, Lc, Ld):
     """Node programs of the wave-per-quartet kernel for one class.
 
@@ -468,29 +480,37 @@ def gen_class(La, Lb, Lc, Ld):
                                              vrr_flop=vflops, hrr_flop=hflops, nodes=len(vrr.nodes))
 
 
-def launcher_source(name, diag, coop=False):
-    out = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.",
-           '#include "%s.h"' % name] + (['#include "coop_%s.h"' % name] if coop else []) + ["", "namespace qf {", "",
-           "void launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
-           " int grid, hipStream_t s)" % name, "{",
-           "    if (nt <= 0) return;",
-           "    if (grid <= 0 || grid > nt) grid = nt;",
-           "    if (fp32)",
-           "        hipLaunchKernelGGL((fock_kernel<%s, float>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
-           "    else",
-           "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
-           "}", "",
-           "void launch_fock_tile_%s(const TileGroup* g, int ng, const Task* t, const PairBlock* b,"
-           " const MolDesc* m, int fp32, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
-           "    if (ng <= 0) return;",
-           "    if (grid <= 0 || grid > ng) grid = ng;",
-           "    if (fp32)",
-           "        hipLaunchKernelGGL((fock_tile_kernel<%s, float>), dim3(grid), dim3(waves * WAVE), lds, s,"
-           " g, ng, t, b, m);" % name,
-           "    else",
-           "        hipLaunchKernelGGL((fock_tile_kernel<%s, double>), dim3(grid), dim3(waves * WAVE), lds, s,"
-           " g, ng, t, b, m);" % name,
-           "}", ""]
+def launcher_source(name, diag, coop=False, lane=True):
+    """Launchers of one class.  ``lane``: the class has per-lane kernels
+    (direct FP64 and LDS-tile FP64/FP32); without them every launch of the
+    class goes to the wave-per-quartet kernel and its Schwarz diagonal to the
+    wave-per-quartet Schwarz kernel."""
+    out = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit."]
+    out += (['#include "%s.h"' % name] if lane else []) + (['#include "coop_%s.h"' % name] if coop else [])
+    out += ["", "namespace qf {", ""]
+    if lane:
+        out += ["// Direct-atomic launch; FP64 only.  The FP32 instantiation of the direct",
+                "// kernel is not built (it gave wrong Fock matrices on gfx950, cause not",
+                "// found): a request for it is refused with a nonzero return.",
+                "int launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
+                " int grid, hipStream_t s)" % name, "{",
+                "    if (fp32) return 1;",
+                "    if (nt <= 0) return 0;",
+                "    if (grid <= 0 || grid > nt) grid = nt;",
+                "    hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
+                "    return 0;",
+                "}", "",
+                "void launch_fock_tile_%s(const TileGroup* g, int ng, const Task* t, const PairBlock* b,"
+                " const MolDesc* m, int fp32, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
+                "    if (ng <= 0) return;",
+                "    if (grid <= 0 || grid > ng) grid = ng;",
+                "    if (fp32)",
+                "        hipLaunchKernelGGL((fock_tile_kernel<%s, float>), dim3(grid), dim3(waves * WAVE), lds, s,"
+                " g, ng, t, b, m);" % name,
+                "    else",
+                "        hipLaunchKernelGGL((fock_tile_kernel<%s, double>), dim3(grid), dim3(waves * WAVE), lds, s,"
+                " g, ng, t, b, m);" % name,
+                "}", ""]
     if coop:
         out += ["void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int grid,"
                 " hipStream_t s)" % name, "{",
@@ -501,10 +521,14 @@ def launcher_source(name, diag, coop=False):
                 "}", ""]
     if diag:
         out += ["void launch_schwarz_%s(PairBlock blk, double* out, hipStream_t s)" % name, "{",
-                "    if (blk.n <= 0) return;",
-                "    hipLaunchKernelGGL((schwarz_kernel<%s>), dim3((blk.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s,"
-                " blk, out);" % name,
-                "}", ""]
+                "    if (blk.n <= 0) return;"]
+        if lane:
+            out += ["    hipLaunchKernelGGL((schwarz_kernel<%s>), dim3((blk.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s,"
+                    " blk, out);" % name]
+        else:
+            out += ["    hipLaunchKernelGGL((schwarz_coop_kernel<%sCoop>), dim3(blk.n), dim3(%sCoop::NT),"
+                    " %sCoop::LDS * sizeof(double), s, blk, out);" % (name, name, name)]
+        out += ["}", ""]
     out += ["}  // namespace qf", ""]
     return "\n".join(out)
 
@@ -535,7 +559,9 @@ def main():
             _write(os.path.join(OUT, "coop_" + name + ".h"),
                    "// Generated by quick_amd/codegen/gen_eri.py - do not edit.\n#pragma once\n"
                    "#include \"../qf_coop.h\"\nnamespace qf {\n" + csrc + "}  // namespace qf\n")
-        _write(os.path.join(OUT, "eri_" + name + ".hip"), launcher_source(name, diag, coop))
+        lane = has_lane_kernels(La, Lb, Lc, Ld)
+        assert lane or coop, "a class needs per-lane or wave-per-quartet kernels"
+        _write(os.path.join(OUT, "eri_" + name + ".hip"), launcher_source(name, diag, coop, lane))
         names.append((name, La, Lb, Lc, Ld, diag))
         info["boys_setup_flop"] = boys_setup_flop(La + Lb + Lc + Ld)
         info["prim_flop"] = info["vrr_flop"] + info["boys_setup_flop"]
@@ -545,16 +571,18 @@ def main():
     pc = {pcl: k for k, pcl in enumerate(PAIR_CLASSES)}
     lines = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.", "#pragma once",
              "#include <hip/hip_runtime.h>", '#include "../qf_types.h"', "", "namespace qf {", "",
-             "using FockLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
+             "// returns nonzero when the launch is refused (FP32 on the direct kernel)",
+             "using FockLaunch = int (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
              "using FockTileLaunch = void (*)(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
              " int, int, size_t, int, hipStream_t);",
              "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);",
              "using SchwarzLaunch = void (*)(PairBlock, double*, hipStream_t);", ""]
     for (name, La, Lb, Lc, Ld, diag) in names:
-        lines.append("void launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
-                     % name)
-        lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
-                     " int, int, size_t, int, hipStream_t);" % name)
+        if has_lane_kernels(La, Lb, Lc, Ld):
+            lines.append("int launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
+                         % name)
+            lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
+                         " int, int, size_t, int, hipStream_t);" % name)
         if (La, Lb, Lc, Ld) in COOP_CLASSES:
             lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);"
                          % name)
@@ -573,13 +601,16 @@ def main():
                      % (pc[(La, Lb)], pc[(Lc, Ld)], info["vrr_flop"] + boys_setup_flop(L),
                         info["hrr_flop"] + 12 * info["ni"]))
     lines += ["    return 0.0;", "}", ""]
-    lines += ["", "// indexed [bra pair class][ket pair class], pair class = La*(La+1)/2 + Lb",
+    lines += ["", "// indexed [bra pair class][ket pair class], pair class = La*(La+1)/2 + Lb;",
+              "// nullptr: the class has no per-lane kernels (wave-per-quartet only)",
               "inline FockLaunch fock_launcher(int cb, int ck)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
-        lines.append("    if (cb == %d && ck == %d) return launch_fock_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
+        if has_lane_kernels(La, Lb, Lc, Ld):
+            lines.append("    if (cb == %d && ck == %d) return launch_fock_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
     lines += ["    return nullptr;", "}", "", "inline FockTileLaunch fock_tile_launcher(int cb, int ck)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
-        lines.append("    if (cb == %d && ck == %d) return launch_fock_tile_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
+        if has_lane_kernels(La, Lb, Lc, Ld):
+            lines.append("    if (cb == %d && ck == %d) return launch_fock_tile_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
     lines += ["    return nullptr;", "}", "", "// wave-per-quartet kernels (nullptr: the class has none)",
               "inline FockCoopLaunch fock_coop_launcher(int cb, int ck)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:

@@ -44,6 +44,9 @@ constexpr double kPi = 3.14159265358979323846264338327950288;
 const double kX0 = 2.0 * std::pow(kPi, 2.5);   // QUICK's X0 (quick_constants_module.f90:61)
 
 inline int pair_class(int La, int Lb) { return La * (La + 1) / 2 + Lb; }
+const char* const kDirectFp32Refused =
+    "the direct-atomic kernels run FP64 only (their FP32 form gave wrong Fock matrices on gfx950 and is not "
+    "built); FP32 halves run on the LDS-tile kernel";
 constexpr int kCoopWindow = 8;   // kets per task of the wave-per-quartet kernels
 const int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
@@ -244,6 +247,9 @@ struct qf_engine {
     };
     std::vector<Launch> launches;
     bool prepared = false;
+    unsigned long long generation = 0;   // bumped by every prepare(): batches check it
+    bool built = false;                  // a build has run since the last prepare()
+    bool count_cm_own = false;           // ... and screened with the engine's own Cutmatrix (d_cm)
     qf_stats stats{};
     double flops_last = -1.0;
     // device
@@ -658,8 +664,10 @@ int prepare(qf_engine* e)
                 // non-finite values, while CPU builds of the same code are right.
                 // (dd|dp) and (dd|dd) stay FP64 (their FP32 forms spill several KB
                 // per lane: little to gain).
-                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 &&
-                                         kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1] < 7;
+                // classes without per-lane kernels (La+Lb+Lc+Ld >= 6) run every
+                // quartet in FP64 on the wave-per-quartet kernel
+                const bool lane_class = qf::fock_launcher(cb, ck) != nullptr;
+                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class;
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
@@ -670,9 +678,10 @@ int prepare(qf_engine* e)
                 // lane costs milliseconds, which dominates the build of a small
                 // molecule
                 const int Lsum = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
-                const bool want_coop =
-                    ct.tile == 2 || (ct.tile < 0 && e->tile != 0 && Lsum >= (e->nbasis <= 128 ? 4 : 6));
+                const bool want_coop = !lane_class || ct.tile == 2 ||
+                                       (ct.tile < 0 && e->tile != 0 && Lsum >= (e->nbasis <= 128 ? 4 : 6));
                 const bool coop = !fp32 && want_coop && qf::fock_coop_launcher(cb, ck) != nullptr;
+                if (!coop && !lane_class) return fail(QF_ERR_STATE, "class has neither per-lane nor wave-per-quartet kernels");
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
                 const int win = coop ? kCoopWindow : qf::WAVE;
@@ -852,6 +861,12 @@ int prepare(qf_engine* e)
     QF_HIP(e->d_counts.resize(2 * std::max<size_t>(1, e->launches.size())));
     e->stats.device_bytes = (long long)(e->d_info.bytes() + e->d_prim.bytes() + e->d_tasks.bytes() +
                                         e->d_Gt.bytes() * 3 + e->d_cm.bytes() + e->d_blocks.bytes());
+    // launch indices recorded for timing refer to the old launch list
+    e->ev_used = 0;
+    e->ev_launch.clear();
+    e->built = false;
+    e->count_cm_own = false;
+    ++e->generation;
     e->prepared = true;
     return QF_OK;
 }
@@ -881,8 +896,6 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n * e->replicas);
     for (size_t li = 0; li < e->launches.size(); ++li) {
         const auto& L = e->launches[li];
-        qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
-        if (!f) return fail(QF_ERR_ARG, "no kernel for class");
         hipEvent_t ea = nullptr, eb = nullptr;
         if (e->timing) {
             while (e->ev_pool.size() < e->ev_used + 2) {
@@ -906,7 +919,10 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
             ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, dmol, L.fp32, L.waves, L.lds,
                e->persist, s);
         } else {
-            f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, e->persist, s);
+            qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
+            if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
+            if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, e->persist, s))
+                return fail(QF_ERR_ARG, kDirectFp32Refused);
         }
         if (e->timing) QF_HIP(hipEventRecord(eb, s));
     }
@@ -926,7 +942,12 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
                            e->d_f1.p, e->d_cm.p);
         dcm = e->d_cm.p;
     }
-    return build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s);
+    const int rc = build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s);
+    if (rc == QF_OK) {
+        e->built = true;
+        e->count_cm_own = (dcm == e->d_cm.p);
+    }
+    return rc;
 }
 
 // Open-shell build (QUICK's get_oshell_eri, gpu_get2e_subs.h OSHELL
@@ -956,7 +977,12 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
     }
     int rc = build_pass(e, e->d_Pt.p, dcm, 1, dFa, dFb, 1.0, 0, s);
     if (rc) return rc;
-    return build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s);
+    rc = build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s);
+    if (rc == QF_OK) {
+        e->built = true;
+        e->count_cm_own = (dcm == e->d_cm.p);
+    }
+    return rc;
 }
 
 }  // namespace
@@ -1211,7 +1237,15 @@ int qf_launch_flops(qf_engine* e, double* flops)
     // Counts, with the density of the last build, the quartets and primitive
     // quartets that pass the screens in every launch, and prices them with
     // the generated per-class FLOP table.
+    // The kernel reads the screen settings and the Cutmatrix of the last
+    // build (device MolDesc slot 0), so it needs a build since the last
+    // prepare, screened with the engine's own Cutmatrix (a caller-owned one
+    // may be gone by now).
     if (!e || !e->prepared) return fail(QF_ERR_STATE, "not prepared");
+    if (!e->built) return fail(QF_ERR_STATE, "no Fock build since the last qf_prepare: counts use its density screen");
+    if (!e->count_cm_own)
+        return fail(QF_ERR_STATE, "the last build screened with a caller-owned Cutmatrix; counts need a build "
+                                  "that computes its own (d_cutmat = NULL)");
     int rc = e->set_device();
     if (rc) return rc;
     const size_t nl = e->launches.size();
@@ -1327,6 +1361,7 @@ int qf_read_timing(qf_engine* e, double* ms_per_launch, int* calls_per_launch)
     }
     if (e->ev_used) QF_HIP(hipEventSynchronize(e->ev_pool[e->ev_used - 1]));
     for (size_t r = 0; r < e->ev_launch.size(); ++r) {
+        if (e->ev_launch[r] < 0 || (size_t)e->ev_launch[r] >= nl) continue;
         float ms = 0.0f;
         QF_HIP(hipEventElapsedTime(&ms, e->ev_pool[2 * r], e->ev_pool[2 * r + 1]));
         ms_per_launch[e->ev_launch[r]] += ms;
@@ -1396,6 +1431,7 @@ __global__ void k_finalize_batch(const BatchMol* __restrict__ bm, int replicas)
 struct qf_batch {
     int device = 0;
     std::vector<qf_engine*> engs;
+    std::vector<unsigned long long> gens;   // each engine's generation when the batch copied its tables
     int replicas = 2;
     int persist = 0;
     std::vector<qf_engine::Launch> launches;
@@ -1460,8 +1496,9 @@ int batch_issue(qf_batch* b, hipStream_t s)
                L.lds, b->persist, s);
         } else {
             qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
-            if (!f) return fail(QF_ERR_ARG, "no kernel for class");
-            f(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, L.fp32, b->persist, s);
+            if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
+            if (f(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, L.fp32, b->persist, s))
+                return fail(QF_ERR_ARG, kDirectFp32Refused);
         }
         if (b->timing) QF_HIP(hipEventRecord(eb, s));
     }
@@ -1499,6 +1536,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         rc = prepare(e);
         if (rc) return bail(rc);
         b->engs.push_back(e);
+        b->gens.push_back(e->generation);
     }
     // pooled buffers
     size_t np = 0, ncm = 0;
@@ -1666,6 +1704,12 @@ int qf_batch_set_graph(qf_batch* b, int enable)
 int qf_batch_build(qf_batch* b, void* stream)
 {
     if (!b) return fail(QF_ERR_ARG, "NULL");
+    // the batch (and its graph) holds pointers into every engine's pair and
+    // task tables: a re-prepared engine has replaced them
+    for (size_t i = 0; i < b->engs.size(); ++i)
+        if (b->engs[i]->generation != b->gens[i] || !b->engs[i]->prepared)
+            return fail(QF_ERR_STATE, "engine " + std::to_string(i) +
+                                          " was changed or re-prepared after the batch was created; create the batch again");
     QF_HIP(hipSetDevice(b->device));
     hipStream_t s = (hipStream_t)stream;
     if (!b->use_graph || b->timing) return batch_issue(b, s);
@@ -1779,6 +1823,7 @@ int qf_batch_read_timing(qf_batch* b, double* ms_per_launch, int* calls_per_laun
     }
     if (b->ev_used) QF_HIP(hipEventSynchronize(b->ev_pool[b->ev_used - 1]));
     for (size_t r = 0; r < b->ev_launch.size(); ++r) {
+        if (b->ev_launch[r] < 0 || (size_t)b->ev_launch[r] >= nl) continue;
         float ms = 0.0f;
         QF_HIP(hipEventElapsedTime(&ms, b->ev_pool[2 * r], b->ev_pool[2 * r + 1]));
         ms_per_launch[b->ev_launch[r]] += ms;

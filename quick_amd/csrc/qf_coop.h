@@ -36,19 +36,21 @@ constexpr int kVU = 2;   // VRR nodes per thread in flight
 constexpr int kHU = 4;   // HRR nodes per thread in flight
 
 
+// Contracted [e0|f0] accumulators of one quartet into s[0, NACC): the
+// primitive loops with QUICK's screen and order, as in contract().  The
+// bra/ket primitive-pair ranges [pb, pe), [qb, qe) select a piece of the
+// quartet.  With primlim < 0 nothing is screened (Schwarz bounds).
 template <class D>
-__device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb, int i, const PairInfo& bi,
-                                             const PairBlock& kb, int j, const PairInfo& ki, double dn,
-                                             const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
-                                             const QF_AS_LDS double* boys_tab)
+__device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, const PairBlock& bb, int i,
+                                              const PairInfo& bi, const PairBlock& kb, int j, const PairInfo& ki,
+                                              double dn, double primlim, QF_AS_LDS double* s,
+                                              const QF_AS_LDS double* boys_tab)
 {
-    constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
     const int lane = threadIdx.x;   // thread index in the workgroup
     QF_AS_LDS double* cf = s + D::CF;   // [0,24) VRR coefficients, 24: 1/2(z+n), [25,31): CD, AB
     for (int x = lane; x < D::NACC; x += D::NT) s[x] = 0.0;
     __syncthreads();
 
-    // primitive loops: QUICK's screen and order, as in contract()
     const auto bp = cptr(bb.prim);
     const auto kp = cptr(kb.prim);
     const int bn = bb.n, bK = bb.K, kn = kb.n, kK = kb.K;
@@ -57,17 +59,17 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
     const auto vrr = gptr(D::vrr());
     const auto vlev = cptr(D::vlev());
     const auto tgt = gptr(D::tgt());
-    for (int jp = t.qb; jp < t.qe; ++jp) {
+    for (int jp = qb; jp < qe; ++jp) {
         const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
-        if (!(c0 * cutk > md.primlim)) break;
+        if (!(c0 * cutk > primlim)) break;
         const double zk = kp[(PF_ZETA * kK + jp) * kn + j];
         const double ozk = kp[(PF_OOZ2 * kK + jp) * kn + j];
         const double Qk[3] = {kp[(PF_PX * kK + jp) * kn + j], kp[(PF_PY * kK + jp) * kn + j],
                               kp[(PF_PZ * kK + jp) * kn + j]};
         const double Kk = kp[(PF_KC * kK + jp) * kn + j];
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
-        for (int ip = t.pb; ip < t.pe; ++ip) {
-            if (!((dn * bp[(PF_CUT * bK + ip) * bn + i]) * cutk > md.primlim)) break;
+        for (int ip = pb; ip < pe; ++ip) {
+            if (!((dn * bp[(PF_CUT * bK + ip) * bn + i]) * cutk > primlim)) break;
             const double zb = bp[(PF_ZETA * bK + ip) * bn + i];
             const double ozb = bp[(PF_OOZ2 * bK + ip) * bn + i];
             const double Pb[3] = {bp[(PF_PX * bK + ip) * bn + i], bp[(PF_PY * bK + ip) * bn + i],
@@ -134,8 +136,15 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
             __syncthreads();
         }
     }
+}
 
-    // HRR on the contracted block (the VRR region is dead now)
+// HRR on the contracted block in s (the VRR region is dead now); the
+// integral of output index x then sits at s[D::out()[x]].
+template <class D>
+__device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki, QF_AS_LDS double* s)
+{
+    const int lane = threadIdx.x;
+    QF_AS_LDS double* cf = s + D::CF;
     if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -164,6 +173,18 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
         }
         __syncthreads();
     }
+}
+
+template <class D>
+__device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb, int i, const PairInfo& bi,
+                                             const PairBlock& kb, int j, const PairInfo& ki, double dn,
+                                             const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
+                                             const QF_AS_LDS double* boys_tab)
+{
+    constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
+    const int lane = threadIdx.x;   // thread index in the workgroup
+    coop_contract<D>(t.pb, t.pe, t.qb, t.qe, bb, i, bi, kb, j, ki, dn, md.primlim, s, boys_tab);
+    coop_hrr<D>(bi, ki, s);
 
     // integrals with cons and degeneracy weight; density blocks
     const double sdeg = 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) * (bi.id != ki.id ? 2.0 : 1.0);
@@ -294,6 +315,42 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
                 coop_quartet<D>(t, bb, i, bi, kb, j, ki, dnv, md, G, s, QF_BOYS_LDS_PTR);
             }
         }
+    }
+}
+
+// Schwarz diagonal of a class (La Lb | La Lb) on the wave-per-quartet node
+// programs: one workgroup per pair of blk, out[j] = max over components
+// |(ab|ab)| with no screening (shellcutoff / classcutoff,
+// quick_cutoff_module.f90:91-424), like schwarz_kernel for the classes that
+// have per-lane kernels.
+template <class D>
+__global__ __launch_bounds__(D::NT) void schwarz_coop_kernel(PairBlock blk, double* __restrict__ out)
+{
+    constexpr int na = ncart(D::La), nb = ncart(D::Lb);
+    static_assert(D::La == D::Lc && D::Lb == D::Ld, "diagonal class");
+    extern __shared__ double qf_lds[];
+    QF_BOYS_LDS_DECL;
+    QF_BOYS_LDS_FILL();
+    QF_AS_LDS double* s = (QF_AS_LDS double*)qf_lds;
+    const int j = blockIdx.x;
+    const PairInfo pi = cptr(blk.info)[j];
+    coop_contract<D>(0, blk.K, 0, blk.K, blk, j, pi, blk, j, pi, 1.0, -1.0, s, QF_BOYS_LDS_PTR);
+    coop_hrr<D>(pi, pi, s);
+    const auto omap = gptr(D::out());
+    double m = 0.0;
+    for (int x = threadIdx.x; x < na * nb; x += D::NT) {
+        const int a = x / nb, b = x % nb;
+        const double cons = kCons[D::La][a] * kCons[D::Lb][b];
+        m = fmax(m, fabs(s[omap[((a * nb + b) * na + a) * nb + b]] * cons * cons));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, WAVE));
+    __syncthreads();   // every lane has read its integrals; reuse s for the reduction
+    if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < D::NT / WAVE; ++w) m = fmax(m, s[w]);
+        out[j] = m;
     }
 }
 

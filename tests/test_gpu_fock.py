@@ -201,21 +201,21 @@ def test_device_scf_matches_host_scf(precision, tol):
 
 
 def test_mixed_precision_error_is_bounded():
-    announce("mixed precision water8 6-31G*")
+    """Mixed precision (FP32 VRR/HRR below 1e-4 of the largest Schwarz
+    product) against the FP64 oracle, water octamer 6-31G*: the error of
+    the Fock matrix and of the two-electron energy in Eh."""
+    announce("mixed precision water8 6-31G* vs oracle")
     mol = molecule("water8")
-    bs = product_basis(mol, "6-31G*")
-    e64 = _engine(bs, cutoff=1e-12)
-    S, _, _ = e64.one_electron()
-    from quick_amd.scf import seeded_density
-    P = seeded_density(S, mol.nelec() // 2)
-    G64 = e64.fock2e(P)
-    emx = _engine(bs, cutoff=1e-12, precision="mixed", split=1e-4)
+    orc = oracle_for(mol, "6-31G*")
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, _ = orc.fock2e(P, cutoff=1e-12, threads=8)
+    emx = _engine(product_basis(mol, "6-31G*"), cutoff=1e-12, precision="mixed", split=1e-4)
     Gmx = emx.fock2e(P)
     st = emx.stats()
     assert st["n_tasks_fp32"] > 0
-    dF = float(np.max(np.abs(Gmx - G64)))
-    dE = 0.5 * abs(float(np.sum(P * (Gmx - G64))))
-    print(f"  mixed: max|dF| {dF:.2e}  |dE2| {dE:.2e} Eh", flush=True)
+    dF = float(np.max(np.abs(Gmx - G_ref)))
+    dE = 0.5 * abs(float(np.sum(P * (Gmx - G_ref))))
+    print(f"  mixed vs oracle: max|dF| {dF:.2e}  |dE2| {dE:.2e} Eh", flush=True)
     assert dF < 1e-6
     assert dE < 1e-6
 
@@ -223,21 +223,46 @@ def test_mixed_precision_error_is_bounded():
 @pytest.mark.parametrize("name", ["water3", "alkane3"])
 def test_mixed_precision_with_direct_fp64_launches(name):
     """Tile digestion off: the FP64 halves run through the direct-atomic
-    kernels while every FP32 half still runs through the LDS-tile kernels."""
-    announce(f"mixed precision, direct FP64 launches, {name} 6-31G*")
+    kernels while every FP32 half still runs through the LDS-tile kernels;
+    checked against the FP64 oracle."""
+    announce(f"mixed precision, direct FP64 launches, {name} 6-31G* vs oracle")
     mol = molecule(name)
-    bs = product_basis(mol, "6-31G*")
-    e64 = _engine(bs, cutoff=1e-10)
-    S, _, _ = e64.one_electron()
-    from quick_amd.scf import seeded_density
-    P = seeded_density(S, mol.nelec() // 2)
-    G64 = e64.fock2e(P)
-    emx = _engine(bs, cutoff=1e-10, precision="mixed")
+    orc = oracle_for(mol, "6-31G*")
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, _ = orc.fock2e(P, cutoff=1e-10, threads=4)
+    emx = _engine(product_basis(mol, "6-31G*"), cutoff=1e-10, precision="mixed")
     emx.set_tile(False)
-    dF = float(np.max(np.abs(emx.fock2e(P) - G64)))
+    G = emx.fock2e(P)
     assert emx.stats()["n_tasks_fp32"] > 0
-    print(f"  max|dF| {dF:.2e}", flush=True)
-    assert dF < 1e-6
+    dF = float(np.max(np.abs(G - G_ref)))
+    dE = 0.5 * abs(float(np.sum(P * (G - G_ref))))
+    print(f"  max|dF| {dF:.2e}  |dE2| {dE:.2e} Eh", flush=True)
+    assert dF < 1e-6 and dE < 1e-6
+
+
+@pytest.mark.skip(reason="the direct-atomic FP32 kernels are not built: the (dp|dp) one gave max|dF| of "
+                         "1.6-3.0 Eh on gfx950 and the (dd|dp) one non-finite values, while the FP64 direct "
+                         "and FP32 LDS-tile kernels of the same classes were right; the cause was not found, so "
+                         "every FP32 half runs on the LDS-tile kernel and a direct FP32 launch is refused")
+def test_direct_fp32_kernels_match_oracle():
+    pass
+
+
+def test_scf_energy_matches_oracle_on_water_cluster():
+    """The water-cluster generator of the 32-water workload at a size the
+    oracle converges in seconds: GPU SCF (FP64 builds) against the oracle's
+    SCF, both with full builds at cutoff 1e-10, to <= 1e-8 Eh."""
+    announce("scf (H2O)4 6-31G* GPU vs oracle")
+    from oracle import oracle as O
+    from quick_amd.scf import rhf
+    mol = molecule("water4")
+    orc = oracle_for(mol, "6-31G*")
+    E_ref, _, _ = O.rhf(orc, mol.nelec(), cutoff=1e-10, prim_limit=1e-11, etol=1e-11, dtol=1e-9, threads=8)
+    eng = _engine(product_basis(mol, "6-31G*"), cutoff=1e-10, prim_limit=1e-11)
+    res = rhf(eng, etol=1e-11, dtol=1e-9)
+    assert res.converged
+    print(f"  E_gpu {res.energy:.12f}  E_oracle {E_ref:.12f}  dE {abs(res.energy - E_ref):.1e}", flush=True)
+    assert abs(res.energy - E_ref) <= 1e-8
 
 
 def test_quick_fortran_abi_drop_in():
