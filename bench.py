@@ -1,23 +1,30 @@
 """Benchmark: two-electron Fock builds per second on one MI355X.
 
-Workload (FP64): RHF/6-31G(d) on a seeded 32-water cluster (96 atoms, 608
-Cartesian functions), QUICK's screening at integral cutoff 1e-11 and
-primitive limit 1e-12 (the values QUICK's adjust_Cutoff ends on near
+Headline (``value``, FP64): RHF/6-31G(d) on a seeded 32-water cluster (96
+atoms, 608 Cartesian functions), QUICK's screening at integral cutoff 1e-11
+and primitive limit 1e-12 (the values QUICK's adjust_Cutoff ends on near
 convergence).  The density is the converged SCF density (SCF run first, its
 wall time and energy reported beside the throughput), resident in HBM.  A
 step is one full Fock build G(P) (screen, all class kernels, symmetrise,
-F += G).
+F += G), replayed from a HIP graph.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|mixed]
 
-Prints one JSON line.  The CPU baseline is the oracle (the C restatement of
-QUICK's CPU get2e) running full builds of the same workload with OpenMP on
-the host cores (median of --cpu-builds), plus a bounded single-thread sample.
+Prints one JSON line.  Beside the headline it carries, under ``north_star``,
+the other workloads of the metric measured in the same run:
+* ``split_precision``: (Gly)10 6-31G(d,p) SCF and builds/s in FP64 and in
+  mixed precision;
+* ``batched``: 256 uneven molecules, plain FP64 per-molecule builds against
+  one batched HIP-graph build in mixed precision, both autotuned;
+* ``serving``: Poisson SCF requests through the continuous-batching server,
+  requests/s and p50/p99 latency at the stated rate.
+The CPU baseline is the oracle (the C restatement of QUICK's CPU get2e) on
+the host cores this job has; the same leg states the GPU results' error in
+Eh against that FP64 oracle (Fock matrices and SCF energies).
 
-Other workloads (--workload): batch (256 uneven molecules per build), serve
-(continuous-batching SCF server), gly10 (split precision), uhf (open-shell
-build) and quick_tests (QUICK's own RHF regression inputs against the build
-times in their saved outputs).
+Single workloads (--workload): batch, serve, gly10, uhf (open-shell build)
+and quick_tests (QUICK's own RHF regression inputs against the build times
+in their saved outputs).
 """
 from __future__ import annotations
 
@@ -41,6 +48,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """CPU threads this job may use: the affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets it (a GPU job's CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +68,7 @@ def main():
     ap.add_argument("--cutoff", type=float, default=1e-11)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--cpu-builds", type=int, default=5, help="oracle builds timed for the CPU baseline (median)")
+    ap.add_argument("--cpu-builds", type=int, default=3, help="oracle builds timed for the CPU baseline (median)")
     ap.add_argument("--workload", default="h2o32", choices=["h2o32", "batch", "serve", "gly10", "uhf", "quick_tests"],
                     help="h2o32: (H2O)32 6-31G(d) builds (default); batch: 256 uneven molecules per build "
                          "(plain FP64 vs batched graph FP64 vs batched graph mixed); serve: Poisson SCF "
@@ -66,6 +83,10 @@ def main():
                     help="primitive pairs per task piece (0: automatic, < 0: never split)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--no-north-star", action="store_true",
+                    help="headline workload only (no split-precision, batched and serving sub-measurements)")
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="default run: serving arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=40, help="default run: serving requests")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus %d is not supported "
@@ -101,6 +122,35 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "copies": 3}))
         return
+    out, ctx = measure_h2o32(a, dev)
+    ns = None
+    if not a.no_north_star:
+        c3, ctx3 = measure_gly10(a, dev)
+        ns = {"split_precision": c3, "batched": measure_batch(a, dev), "serving": measure_serve(a, dev)}
+        ctx["gly10"] = ctx3
+    cpu = None if a.no_cpu_baseline else cpu_baseline_leg(a, ctx)
+    if cpu is not None:
+        checks = cpu.pop("checks")
+        if out["scf"] is not None and "c2_scf_energy_oracle_eh" in checks:
+            out["scf"]["oracle_energy_eh"] = checks["c2_scf_energy_oracle_eh"]
+            out["scf"]["dE_vs_oracle_eh"] = checks["c2_dE_scf_vs_oracle_eh"]
+        if ns is not None and "c3" in checks:
+            ns["split_precision"]["vs_fp64_oracle"] = checks["c3"]
+        out["vs_fp64_oracle"] = {k[3:]: v for k, v in checks.items() if k.startswith("c2_")}
+    out["north_star"] = ns
+    out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def measure_h2o32(a, dev):
+    """The headline workload: (H2O)32 6-31G(d) Fock builds on resident
+    buffers (graph replay), after the SCF that produces the density."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.engine import FockBatch, FockEngine
+    from quick_amd.scf import rhf, rhf_device
+    from quick_amd.scf import seeded_density as seeded_density_np
     mol = M.water_cluster(a.nwater)
     bs = B.build_basis(mol, "6-31G*")
     log(f"[bench] (H2O){a.nwater} 6-31G(d): {mol.natom} atoms, {bs.nbasis} functions")
@@ -123,7 +173,6 @@ def main():
     scf = None
     S, T, V = eng.one_electron()
     if not a.no_scf:
-        from quick_amd.scf import rhf_device
         eng.fock2e(seeded_density_np(S, mol.nelec() // 2))          # load the kernels once
         res = rhf(eng, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
         log(f"[bench] SCF (plain) E = {res.energy:.10f} in {res.iterations} it, {res.wall:.2f}s")
@@ -162,6 +211,7 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] autotune {time.perf_counter() - ta:.2f}s: " +
             ", ".join(f"{k}:{v[0]}" for k, v in sorted(tuned.items())))
+
     def timed(fn, steps):
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -180,7 +230,6 @@ def main():
     ms_eager, _ = timed(eager, a.steps)
     # the same build replayed from a HIP graph (a one-molecule qf_batch over
     # the same resident P and F buffers: no launch gaps between classes)
-    from quick_amd.engine import FockBatch
     gb = FockBatch([eng], replicas=eng_replicas, graph=True, density_ptrs=[P.data_ptr()], fock_ptrs=[F.data_ptr()])
     graph = lambda: gb.build(sh)
     for _ in range(a.warmup):
@@ -225,37 +274,6 @@ def main():
         if tj.get("kernel") == kname:
             traffic = tj["bytes_per_launch"]
 
-    # ---- CPU baseline: the oracle, one full build on the host cores ---------
-    cpu = None
-    if not a.no_cpu_baseline:
-        from oracle import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        qb = O.QuickBasisRef(mol.symbols, mol.xyz, "6-31G*")
-        orc = O.Oracle(qb)
-        times = []
-        for _ in range(max(1, a.cpu_builds)):
-            c0 = time.perf_counter()
-            G_cpu, nq = orc.fock2e(P_host, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, threads=threads)
-            times.append(time.perf_counter() - c0)
-            log(f"[bench] CPU oracle build {times[-1]:.2f}s on {threads} threads")
-        tcpu = float(np.median(times))
-        dF = float(np.max(np.abs(G_graph - G_cpu)))
-        # one thread: the serial restatement over the first eighth of the bra
-        # shells, extrapolated to the whole build by its share of quartets
-        nsh = qb.nshell
-        c0 = time.perf_counter()
-        _, nq1 = orc.fock2e_shell_range(P_host, 0, max(1, nsh // 8), cutoff=a.cutoff, prim_limit=0.1 * a.cutoff)
-        t1 = time.perf_counter() - c0
-        t1_full = t1 * nq / max(1, nq1)
-        log(f"[bench] CPU oracle 1 thread: {nq1} of {nq} quartets in {t1:.2f}s -> {t1_full:.1f}s per build")
-        cpu = {"value": round(1.0 / tcpu, 5), "unit": "Fock builds/s", "cores": threads, "kind": "port",
-               "sample": f"median of {len(times)} full (H2O){a.nwater} 6-31G(d) builds of the same density "
-                         f"({nq} shell quartets each, {tcpu:.2f}s per build, OpenMP over bra shells); "
-                         f"max|dF| GPU vs CPU {dF:.1e}",
-               "single_thread": {"value": round(1.0 / t1_full, 5), "unit": "Fock builds/s", "cores": 1,
-                                 "sample": f"bra shells 0..{max(1, nsh // 8) - 1} of {nsh} ({nq1} of {nq} quartets, "
-                                           f"{t1:.2f}s), extrapolated by quartet count"}}
-
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -280,10 +298,200 @@ def main():
         "build_gflop": round(build_flops / 1e9, 3),
         "wall_s_timed": round(wall, 3),
         "scf": scf,
+        "per_class": per_class,
         "autotuned": {k: list(v[0]) for k, v in sorted(tuned.items())} if tuned else None,
-        "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    ctx = {"mol": mol, "P": P_host, "G": G_graph, "E_scf": scf["energy_eh"] if scf else None}
+    return out, ctx
+
+
+def measure_gly10(a, dev):
+    """Split precision on (Gly)10 6-31G(d,p) (73 atoms, 775 functions): SCF
+    with full builds in FP64 and in mixed precision (quartets with Schwarz
+    product < 1e-4 max in FP32), then builds/s of both on the FP64-converged
+    density, resident buffers, autotuned."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.autotune import autotune
+    from quick_amd.engine import FockEngine
+    from quick_amd.scf import rhf
+    mol = M.polyglycine(10)
+    bs = B.build_basis(mol, "6-31G**")
+    sh = torch.cuda.current_stream(dev).cuda_stream
+    res, engs = {}, {}
+    for prec in ("fp64", "mixed"):
+        e = FockEngine(bs, cutoff=1e-11, prim_limit=1e-12, precision=prec)
+        S, T, V = e.one_electron()
+        r = rhf(e, etol=1e-8, dtol=1e-6, one_electron=(S, T, V))
+        log(f"[gly10] SCF {prec}: E = {r.energy:.10f} in {r.iterations} it, {r.wall:.2f}s")
+        res[prec], engs[prec] = r, e
+    P64 = res["fp64"].density
+    Pd = torch.from_numpy(np.ascontiguousarray(P64)).to(dev)
+    Fd = torch.zeros_like(Pd)
+    rate, G = {}, {}
+    for prec, e in engs.items():
+        if a.autotune:
+            autotune(e, sh, builds=2, make_density=lambda i: Pd.data_ptr())
+        t = _timed(lambda: e.fock2e_device(Pd.data_ptr(), Fd.data_ptr(), sh), a.steps, a.warmup)
+        rate[prec] = 1.0 / t
+        G[prec] = e.fock2e(P64)
+        log(f"[gly10] {prec}: {1e3 * t:.3f} ms/build")
+    st = engs["mixed"].stats()
+    out = {"workload": "(Gly)10 beta strand RHF/6-31G(d,p) Fock build, split 1e-4 of max Schwarz product",
+           "natom": mol.natom, "nbasis": bs.nbasis, "integral_cutoff": 1e-11,
+           "fp64_builds_per_s": round(rate["fp64"], 3), "mixed_builds_per_s": round(rate["mixed"], 3),
+           "mixed_over_fp64": round(rate["mixed"] / rate["fp64"], 3),
+           "fp32_task_share": round(st["n_tasks_fp32"] / max(1, st["n_tasks_fp32"] + st["n_tasks_fp64"]), 3),
+           "scf_energy_fp64_eh": res["fp64"].energy, "scf_energy_mixed_eh": res["mixed"].energy,
+           "dE_scf_mixed_vs_fp64_eh": abs(res["mixed"].energy - res["fp64"].energy),
+           "max_abs_dF_mixed_vs_fp64_eh": float(np.max(np.abs(G["mixed"] - G["fp64"]))),
+           "scf_wall_s": {k: round(v.wall, 3) for k, v in res.items()},
+           "scf_iterations": {k: v.iterations for k, v in res.items()}}
+    for e in engs.values():
+        e.close()
+    ctx = {"mol": mol, "P": P64, "G": G, "E": {k: v.energy for k, v in res.items()}}
+    return out, ctx
+
+
+def measure_batch(a, dev):
+    """256 uneven molecules (water clusters 1-16, n-alkanes C1-C12),
+    6-31G(d): all Fock builds of one SCF iteration.  Plain FP64 (one
+    molecule after another, each build on its own) against one batched
+    build replayed from a HIP graph, in FP64 and in mixed precision; the
+    launch shapes are autotuned on the batch and the plain builds use the
+    same engines, so both run under the same tuning."""
+    import torch
+    from quick_amd import basis as B
+    from quick_amd import molecules as M
+    from quick_amd.autotune import autotune
+    from quick_amd.engine import FockBatch, FockEngine
+    from quick_amd.scf import seeded_density
+    mols = M.mixed_set(a.molecules)
+    t0 = time.perf_counter()
+    e64, emx, dens = [], [], []
+    for m in mols:
+        bs = B.build_basis(m, "6-31G*")
+        e = FockEngine(bs, cutoff=1e-10)
+        S, _, _ = e.one_electron()
+        dens.append(seeded_density(S, m.nelec() // 2))
+        e64.append(e)
+        emx.append(FockEngine(bs, cutoff=1e-10, precision="mixed"))
+    nb = [e.n for e in e64]
+    log(f"[batch] {len(mols)} molecules, nbasis {min(nb)}..{max(nb)}, setup {time.perf_counter() - t0:.1f}s")
+    s = torch.cuda.current_stream(dev).cuda_stream
+    res = {}
+    for name, engs in (("fp64", e64), ("mixed", emx)):
+        if a.autotune:
+            autotune(engs, s, builds=1, batch_kw={"replicas": 2}, densities=dens)
+        b = FockBatch(engs, replicas=2, graph=True)
+        for i, P in enumerate(dens):
+            b.set_density(i, P)
+        res[name] = (_timed(lambda: b.build(s), a.steps, a.warmup), b)
+    # plain FP64: every molecule on its own, as a per-molecule drop-in would,
+    # on the FP64 engines (tuned above)
+    Pd = [torch.from_numpy(P).to(dev) for P in dens]
+    Fd = [torch.zeros_like(p) for p in Pd]
+
+    def plain():
+        for e, p, f in zip(e64, Pd, Fd):
+            e.fock2e_device(p.data_ptr(), f.data_ptr(), s)
+    t_plain = _timed(plain, max(2, a.steps // 4), 1)
+    dF, dE = 0.0, 0.0
+    b64, bmx = res["fp64"][1], res["mixed"][1]
+    for i, P in enumerate(dens):
+        G64, _ = b64.get(i)
+        Gmx, _ = bmx.get(i)
+        dF = max(dF, float(np.max(np.abs(Gmx - G64))))
+        dE = max(dE, 0.5 * abs(float(np.sum(P * (Gmx - G64)))))
+    for _, b in res.values():
+        b.close()
+    nm = len(mols)
+    log(f"[batch] plain {nm / t_plain:.1f}, batched fp64 {nm / res['fp64'][0]:.1f}, "
+        f"batched mixed {nm / res['mixed'][0]:.1f} builds/s")
+    return {"workload": f"{nm} molecules (water clusters 1-16, n-alkanes C1-C12) RHF/6-31G(d), all Fock builds "
+                        "of one SCF iteration; integral cutoff 1e-10, seeded densities",
+            "nbasis_min": min(nb), "nbasis_max": max(nb), "nbasis_sum": int(sum(nb)),
+            "plain_fp64_builds_per_s": round(nm / t_plain, 3),
+            "batched_fp64_graph_builds_per_s": round(nm / res["fp64"][0], 3),
+            "batched_mixed_graph_builds_per_s": round(nm / res["mixed"][0], 3),
+            "batched_mixed_over_plain_fp64": round(t_plain / res["mixed"][0], 3),
+            "batched_fp64_over_plain_fp64": round(t_plain / res["fp64"][0], 3),
+            "mixed_max_abs_dF_vs_fp64_eh": dF, "mixed_max_abs_dE2_vs_fp64_eh": dE}
+
+
+def measure_serve(a, dev):
+    """Poisson SCF requests (molecules of 3-60 atoms from the batch generator,
+    basis uniform over 6-31G / 6-31G(d) / def2-SVP, mixed precision) through
+    the continuous-batching server."""
+    from quick_amd.serve import FockServer, request_stream, summarize
+    srv = FockServer(dev, precision="mixed")
+    srv.run(request_stream(1, 1000.0, seed=7))          # warm-up request
+    results, st = srv.run(request_stream(a.serve_requests, a.serve_rate))
+    sm = summarize(results, st["wall_s"])
+    log(f"[serve] {sm}")
+    return {"workload": f"{a.serve_requests} Poisson RHF SCF requests, molecules 3-60 atoms, basis in "
+                        "{6-31G, 6-31G(d), def2-SVP}, mixed precision, continuous batching",
+            "rate_per_s": a.serve_rate, "requests_per_s": round(sm["requests_per_s"], 4),
+            "p50_latency_s": round(sm["p50_s"], 4), "p99_latency_s": round(sm["p99_s"], 4),
+            "converged": sm["converged"], "requests": sm["requests"], "wall_s": round(st["wall_s"], 3),
+            "server_steps": st["steps"]}
+
+
+def cpu_baseline_leg(a, ctx):
+    """The CPU baseline: the oracle (C restatement of QUICK's CPU get2e) on
+    this job's host threads, timing full builds of the headline workload's
+    density (median of --cpu-builds) and one full single-thread build.  The
+    same FP64 oracle then states the GPU results' error in Eh: the headline
+    Fock matrix, the SCF fixed point (the oracle's SCF from the GPU's
+    converged density) and, for the split-precision workload, both GPU Fock
+    matrices and both SCF energies."""
+    from oracle import oracle as O
+    threads = host_threads()
+    mol = ctx["mol"]
+    P_host = ctx["P"]
+    qb = O.QuickBasisRef(mol.symbols, mol.xyz, "6-31G*")
+    orc = O.Oracle(qb)
+    times = []
+    for _ in range(max(1, a.cpu_builds)):
+        c0 = time.perf_counter()
+        G_cpu, nq = orc.fock2e(P_host, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, threads=threads)
+        times.append(time.perf_counter() - c0)
+        log(f"[cpu] oracle build {times[-1]:.2f}s on {threads} threads")
+    tcpu = float(np.median(times))
+    dF = float(np.max(np.abs(ctx["G"] - G_cpu)))
+    c0 = time.perf_counter()
+    orc.fock2e(P_host, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, threads=1)
+    t1 = time.perf_counter() - c0
+    log(f"[cpu] oracle build {t1:.2f}s on 1 thread")
+    checks = {"c2_max_abs_dF_gpu_vs_oracle_eh": dF}
+    if ctx.get("E_scf") is not None:
+        nel = mol.nelec()
+        E_o, _, it = O.rhf(orc, nel, cutoff=a.cutoff, prim_limit=0.1 * a.cutoff, etol=1e-8, dtol=1e-6,
+                           threads=threads, guess=P_host)
+        checks["c2_scf_energy_oracle_eh"] = float(E_o)
+        checks["c2_dE_scf_vs_oracle_eh"] = abs(float(E_o) - ctx["E_scf"])
+        log(f"[cpu] oracle SCF from the GPU density: E = {E_o:.10f} ({len(it)} it), dE {checks['c2_dE_scf_vs_oracle_eh']:.1e}")
+    g = ctx.get("gly10")
+    if g is not None:
+        m3 = g["mol"]
+        o3 = O.Oracle(O.QuickBasisRef(m3.symbols, m3.xyz, "6-31G**"))
+        G3, _ = o3.fock2e(g["P"], cutoff=1e-11, prim_limit=1e-12, threads=threads)
+        E3, _, it3 = O.rhf(o3, m3.nelec(), cutoff=1e-11, prim_limit=1e-12, etol=1e-8, dtol=1e-6, threads=threads,
+                           guess=g["P"])
+        checks["c3"] = {"max_abs_dF_fp64_eh": float(np.max(np.abs(g["G"]["fp64"] - G3))),
+                        "max_abs_dF_mixed_eh": float(np.max(np.abs(g["G"]["mixed"] - G3))),
+                        "dE2_mixed_eh": 0.5 * abs(float(np.sum(g["P"] * (g["G"]["mixed"] - G3)))),
+                        "scf_energy_oracle_eh": float(E3),
+                        "dE_scf_fp64_eh": abs(g["E"]["fp64"] - float(E3)),
+                        "dE_scf_mixed_eh": abs(g["E"]["mixed"] - float(E3))}
+        log(f"[cpu] (Gly)10 vs oracle: {checks['c3']}")
+    return {"value": round(1.0 / tcpu, 5), "unit": "Fock builds/s", "cores": threads, "kind": "port",
+            "sample": f"median of {len(times)} full (H2O){a.nwater} 6-31G(d) builds of the same density "
+                      f"({nq} shell quartets each, {tcpu:.2f}s per build, OpenMP over bra shells)",
+            "single_thread": {"value": round(1.0 / t1, 5), "unit": "Fock builds/s", "cores": 1,
+                              "sample": f"one full build, {t1:.2f}s"},
+            "checks": checks}
 
 
 def _timed(fn, steps, warmup):

@@ -85,7 +85,7 @@ ELEMENTS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne",
             "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar"]
 _FILES = {"sto-3g": "STO-3G", "3-21g": "3-21G", "6-31g": "6-31G", "6-31g*": "6-31Gs",
           "6-31g(d)": "6-31Gs", "6-31g**": "6-31Gss", "6-31g(d,p)": "6-31Gss",
-          "cc-pvdz": "cc-pVDZ", "def2-svp": "def2-SVP"}
+          "cc-pvdz": "cc-pVDZ", "def2-svp": "def2-SVP", "cc-pvtz": "cc-pVTZ", "def2-tzvp": "def2-TZVP"}
 
 
 def xnorm(a, i, j, k):
@@ -171,6 +171,13 @@ class QuickBasisRef:
                     w2 = xnewnorm(1, 1, 0, raw[5], AA)     # from the zz function
                     funcs = [[w1 * x for x in raw[k]] for k in range(3)] + \
                             [[w2 * x for x in raw[k]] for k in range(3, 6)]
+                elif t == "F":
+                    # basis.f90:652-720: gccoeff = coefficient * xnorm(KLMN) per
+                    # component, no contraction rescale (no xnewnorm for F)
+                    qstart.append(3); qfinal.append(3)
+                    klmn = [(3, 0, 0), (2, 1, 0), (1, 2, 0), (0, 3, 0), (2, 0, 1), (1, 1, 1), (0, 2, 1),
+                            (1, 0, 2), (0, 1, 2), (0, 0, 3)]
+                    funcs = [[b * xnorm(a, *k) for a, b in zip(AA, sh["coefs"][0])] for k in klmn]
                 else:
                     raise ValueError("shell type " + t)
                 per_shell_coef.append(funcs)
@@ -310,9 +317,10 @@ def seeded_density(S, nocc, seed=20261202):
 
 
 def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=1e-10,
-        dtol=1e-8, ndiis=10, verbose=False, threads=1):
+        dtol=1e-8, ndiis=10, verbose=False, threads=1, guess=None):
     """Plain-DIIS RHF with full (non-incremental) Fock builds by the oracle
-    (``threads`` > 1: the OpenMP build, equal to the serial one)."""
+    (``threads`` > 1: the OpenMP build, equal to the serial one).  The first
+    density is the core-Hamiltonian guess, or ``guess``."""
     qb = oracle.qb
     S, T, V = oracle.one_electron()
     H = T + V
@@ -326,7 +334,7 @@ def rhf(oracle: Oracle, nelec, cutoff=1e-10, prim_limit=None, maxiter=100, etol=
         C = X @ Cp
         return 2.0 * C[:, :nocc] @ C[:, :nocc].T
 
-    P = density(H)
+    P = density(H) if guess is None else np.array(guess, dtype=np.float64)
     E_old = 0.0
     fs, es = [], []
     hist = []

@@ -31,10 +31,10 @@
 #endif
 
 #define QO_PI 3.14159265358979323846264338327950288
-#define MAXAM 4            /* highest AM on one side: d+d */
+#define MAXAM 6            /* highest AM on one side: f+f */
 #define NCUM(L) ((L) < 0 ? 0 : (((L) + 1) * ((L) + 2) * ((L) + 3) / 6))
-#define NCMAX 35           /* NCUM(4) */
-#define MMAX 9             /* m = 0..8 */
+#define NCMAX 84           /* NCUM(6) */
+#define MMAX 13            /* m = 0..12 */
 
 /* ------------------------------------------------------------------------- */
 /* Cartesian component tables                                                */
@@ -61,6 +61,15 @@ static void init_tables(void)
 /* QUICK's per-shell component order and cons (basis.f90:567-640). */
 static const int q_comp_p[3][3] = {{1,0,0},{0,1,0},{0,0,1}};
 static const int q_comp_d[6][3] = {{2,0,0},{1,1,0},{0,2,0},{1,0,1},{0,1,1},{0,0,2}};
+static const int q_comp_f[10][3] = {{3,0,0},{2,1,0},{1,2,0},{0,3,0},{2,0,1},{1,1,1},{0,2,1},{1,0,2},{0,1,2},{0,0,3}};
+/* cons of the f components (basis.f90:652-720) */
+static double q_cons_f(int k)
+{
+    static const double c[10] = {1.0, 2.2360679774997896964, 2.2360679774997896964, 1.0, 2.2360679774997896964,
+                                 3.8729833462074168852, 2.2360679774997896964, 2.2360679774997896964,
+                                 2.2360679774997896964, 1.0};
+    return c[k];
+}
 
 /* ------------------------------------------------------------------------- */
 struct qo_ctx {
@@ -226,7 +235,7 @@ static void vrr_full(int E, int F, const double *fm, double prefac, const vrr_pa
 /* (a b| = sum_{t<=b} prod_d C(b_d,t_d) (A_d-B_d)^(b_d-t_d) (a+t 0|.        */
 static int hrr_terms(const int *a, const int *b, const double *AB, double *coef, int *ang)
 {
-    static const double binom[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 2, 1}};
+    static const double binom[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
     int n = 0;
     for (int tx = 0; tx <= b[0]; ++tx)
         for (int ty = 0; ty <= b[1]; ++ty)
@@ -325,11 +334,12 @@ static void build_function_tables(qo_ctx *c)
             c->qfbasis[s * 4 + L] = off + nc - 1;
             for (int k = 0; k < nc; ++k) {
                 int f = c->ksumtype[s] + off + k;
-                const int *lmn = L == 0 ? (const int[3]){0, 0, 0} : (L == 1 ? q_comp_p[k] : q_comp_d[k]);
+                const int *lmn = L == 0 ? (const int[3]){0, 0, 0}
+                                 : (L == 1 ? q_comp_p[k] : (L == 2 ? q_comp_d[k] : q_comp_f[k]));
                 c->fn_lmn[f * 3 + 0] = lmn[0];
                 c->fn_lmn[f * 3 + 1] = lmn[1];
                 c->fn_lmn[f * 3 + 2] = lmn[2];
-                c->fn_cons[f] = (L == 2 && (k == 1 || k == 3 || k == 4)) ? sqrt(3.0) : 1.0;
+                c->fn_cons[f] = L == 3 ? q_cons_f(k) : ((L == 2 && (k == 1 || k == 3 || k == 4)) ? sqrt(3.0) : 1.0);
             }
             off += nc;
         }
@@ -410,8 +420,8 @@ static void contract(const qo_ctx *c, const prim_store *ps, int I, int J, int K,
 static double hrr_whole(const qo_ctx *c, int III, int JJJ, int KKK, int LLL, const double *AB,
                         const double *CD, double store[NCMAX][NCMAX])
 {
-    double cl[32], cr[32];
-    int al[32], ar[32];
+    double cl[64], cr[64];
+    int al[64], ar[64];
     int nl = hrr_terms(c->fn_lmn + 3 * III, c->fn_lmn + 3 * JJJ, AB, cl, al);
     int nr = hrr_terms(c->fn_lmn + 3 * KKK, c->fn_lmn + 3 * LLL, CD, cr, ar);
     double Y = 0.0;
@@ -927,7 +937,7 @@ void qo_eri_tensor(const qo_ctx *c, double *eri)
 /* QUICK: quick_overlap_module.f90 (overlap), quick_oei_module.f90:69-257    */
 /* (kinetic, attrashell nuclear attraction with FmT).                        */
 /* ------------------------------------------------------------------------- */
-static void os_1d_overlap(int la, int lb, double PA, double PB, double oo2p, double S[7][7])
+static void os_1d_overlap(int la, int lb, double PA, double PB, double oo2p, double S[8][8])
 {
     S[0][0] = 1.0;
     for (int i = 1; i <= la; ++i) {
@@ -975,7 +985,7 @@ void qo_one_electron(const qo_ctx *c, double *Smat, double *Tmat, double *Vmat)
                         AB2 += (A[d] - B[d]) * (A[d] - B[d]);
                     }
                     double Kab = exp(-al * be / p * AB2);
-                    double S1[3][7][7];
+                    double S1[3][8][8];
                     for (int d = 0; d < 3; ++d) os_1d_overlap(a[d] + 1, b[d] + 2, PA[d], PB[d], oo2p, S1[d]);
                     double s0 = pow(QO_PI / p, 1.5) * Kab;
                     double Sx = S1[0][a[0]][b[0]], Sy = S1[1][a[1]][b[1]], Sz = S1[2][a[2]][b[2]];
@@ -1007,7 +1017,7 @@ void qo_one_electron(const qo_ctx *c, double *Smat, double *Tmat, double *Vmat)
                         double fm[16];
                         qo_fmt(maxm, p * PC2, fm);
                         /* table over (a-comp, b-comp, m): use lmn up to la/lb */
-                        double V[3][3][3][3][3][3][5];
+                        static __thread double V[4][4][4][4][4][4][7];
                         /* V[ax][ay][az][bx][by][bz][m]; fill base */
                         for (int m = 0; m <= L; ++m) V[0][0][0][0][0][0][m] = 2.0 * QO_PI / p * Kab * fm[m];
                         /* build a (b = 0) */

@@ -39,8 +39,11 @@ def _class_times(build, launches, read_timing, set_timing, builds):
     return out
 
 
+NPC = 10    # pair classes s..f
+
+
 def _class_index(name):
-    names = ["ss", "ps", "pp", "ds", "dp", "dd"]
+    names = ["ss", "ps", "pp", "ds", "dp", "dd", "fs", "fp", "fd", "ff"]
     a, b = name.strip("()").split("|")
     return names.index(a), names.index(b)
 
@@ -73,7 +76,7 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
     times = {}
     for cand in candidates:
         for e in engs:
-            for cb in range(6):
+            for cb in range(NPC):
                 for ck in range(cb + 1):
                     e.set_class_tuning(cb, ck, *cand)
         times[cand] = timed()
@@ -95,7 +98,7 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
     ptimes = {}
     for pc in pieces or []:
         for e in engs:
-            for cb in range(6):
+            for cb in range(NPC):
                 for ck in range(cb + 1):
                     e.set_class_piece(cb, ck, pc)
         ptimes[pc] = timed()

@@ -6,12 +6,15 @@ Follows QUICK's reader and normalisation (``src/basis.f90:370-720``,
 * shells are kept per atom in file order; an ``SP`` shell is *one* shell with
   ``Qstart=0, Qfinal=1`` that shares its exponents between an s and a p part
   (``basis.f90:476-540``);
-* Cartesian components are ordered s; x, y, z; xx, xy, yy, xz, yz, zz
-  (``basis.f90:567-640`` KLMN);
+* Cartesian components are ordered s; x, y, z; xx, xy, yy, xz, yz, zz;
+  xxx, xxy, xyy, yyy, xxz, xyz, yyz, xzz, yzz, zzz (``basis.f90:567-720``
+  KLMN);
 * every Cartesian function is normalised to one.  The contraction
   coefficient stored per (shell, L) is the one of the axis-aligned function
   x^L (QUICK's ``gccoeff`` of function ``ksumtype+L``); mixed components carry
-  the extra factor ``cons`` (sqrt(3) for xy, xz, yz), as QUICK's ``cons``.
+  the extra factor ``cons`` (sqrt(3) for xy, xz, yz; sqrt(5) for the f
+  components with one exponent 1 and one 2, sqrt(15) for xyz), as QUICK's
+  ``cons``.  f contractions are not renormalised (QUICK's reader does not).
 
 Coordinates are converted from Angstrom with QUICK's BOHR = 0.52917720859
 (``src/modules/quick_constants_module.f90:65``).
@@ -37,17 +40,21 @@ _ALIASES = {
     "sto-3g": "STO-3G", "3-21g": "3-21G", "6-31g": "6-31G",
     "6-31g*": "6-31Gs", "6-31g(d)": "6-31Gs",
     "6-31g**": "6-31Gss", "6-31g(d,p)": "6-31Gss",
-    "cc-pvdz": "cc-pVDZ", "def2-svp": "def2-SVP",
+    "cc-pvdz": "cc-pVDZ", "def2-svp": "def2-SVP", "cc-pvtz": "cc-pVTZ", "def2-tzvp": "def2-TZVP",
 }
 
 CART = {
     0: [(0, 0, 0)],
     1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
     2: [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)],
+    3: [(3, 0, 0), (2, 1, 0), (1, 2, 0), (0, 3, 0), (2, 0, 1), (1, 1, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+        (0, 0, 3)],
 }
+_S3, _S5 = math.sqrt(3.0), math.sqrt(5.0)
 CONS = {0: [1.0], 1: [1.0, 1.0, 1.0],
-        2: [1.0, math.sqrt(3.0), 1.0, math.sqrt(3.0), math.sqrt(3.0), 1.0]}
-_SHELL_L = {"S": (0, 0), "P": (1, 1), "SP": (0, 1), "D": (2, 2)}
+        2: [1.0, _S3, 1.0, _S3, _S3, 1.0],
+        3: [1.0, _S5, _S5, 1.0, _S5, _S5 * _S3, _S5, _S5, _S5, 1.0]}
+_SHELL_L = {"S": (0, 0), "P": (1, 1), "SP": (0, 1), "D": (2, 2), "F": (3, 3)}
 
 
 def _dfact(n):
@@ -61,13 +68,17 @@ def _dfact(n):
 def normalized_coefficients(exps, coefs, L):
     """Coefficients of the normalised x^L contracted function.
 
-    Primitive factor (2a)^(L/2+3/4) pi^(-3/4) ((2L-1)!!/2^L)^(-1/2), then the
-    contraction is rescaled to unit self-overlap (``xnorm`` and ``xnewnorm``).
+    Primitive factor (2a)^(L/2+3/4) pi^(-3/4) ((2L-1)!!/2^L)^(-1/2), then,
+    for s, p and d, the contraction is rescaled to unit self-overlap
+    (``xnorm`` and ``xnewnorm``).  QUICK's reader applies no contraction
+    rescale to f shells (basis.f90:652-720), so neither does this.
     """
     a = np.asarray(exps, dtype=np.float64)
     d = np.asarray(coefs, dtype=np.float64)
     g = _dfact(2 * L - 1) / (2.0 ** L)
     c = d * (2.0 * a) ** (L / 2.0 + 0.75) * math.pi ** (-0.75) * g ** (-0.5)
+    if L >= 3:
+        return c
     s = np.sum(np.outer(c, c) * (a[:, None] + a[None, :]) ** (-1.5 - L))
     s = (s * g * math.pi ** 1.5) ** (-0.5)
     return c * s
@@ -182,7 +193,7 @@ def build_basis(mol: Molecule, name: str) -> BasisSet:
             raise ValueError(f"basis {name} has no entry for {sym}")
         for sh in shells:
             if sh["type"] not in _SHELL_L:
-                raise ValueError(f"shell type {sh['type']} is beyond d (not supported)")
+                raise ValueError(f"shell type {sh['type']} is beyond f (not supported)")
             lmin, lmax = _SHELL_L[sh["type"]]
             coef = {}
             for k, L in enumerate(range(lmin, lmax + 1)):

@@ -1,8 +1,9 @@
 """Generate the per-class ERI recursions for the HIP kernels.
 
 For every canonical class (La Lb | Lc Ld) with La >= Lb, Lc >= Ld, and bra
-pair class >= ket pair class (21 classes for s, p, d) this emits a struct
-with two straight-line device functions:
+pair class >= ket pair class (55 classes for s, p, d, f) this emits, for the
+classes that have per-lane kernels, a struct with two straight-line device
+functions:
 
 * ``vrr`` - Obara-Saika / Head-Gordon-Pople vertical recursion for one
   primitive quartet: builds [e0|f0]^(0) for every Cartesian e with
@@ -28,10 +29,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(os.path.dirname(HERE), "csrc", "gen")
 
+# QUICK's Cartesian component order per shell (basis.f90:567-720 KLMN):
+# by ascending z power, then descending x power.
 QCOMP = {
     0: [(0, 0, 0)],
     1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
     2: [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)],
+    3: [(3, 0, 0), (2, 1, 0), (1, 2, 0), (0, 3, 0), (2, 0, 1), (1, 1, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+        (0, 0, 3)],
 }
 
 
@@ -250,11 +255,25 @@ class Hrr:
         return lines, flops
 
 
-# Classes that also get a wave-per-quartet kernel (fock_coop_kernel): the
-# VRR and HRR run as level-scheduled node programs over LDS, so a quartet's
-# intermediates never leave the chip.
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 0), (3, 1), (3, 2), (3, 3)]
+
+
+def quartet_classes():
+    out = []
+    for kb, (La, Lb) in enumerate(PAIR_CLASSES):
+        for kk, (Lc, Ld) in enumerate(PAIR_CLASSES):
+            if kk <= kb:
+                out.append((La, Lb, Lc, Ld))
+    return out
+
+
+# Classes that get a wave-per-quartet kernel (fock_coop_kernel): the VRR and
+# HRR run as level-scheduled node programs over LDS, so a quartet's
+# intermediates never leave the chip.  Every class with an f shell has only
+# this kernel.
 COOP_CLASSES = [(1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0),
                 (2, 2, 1, 0), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+COOP_CLASSES += [c for c in quartet_classes() if max(c) == 3]
 
 
 # Classes with La+Lb+Lc+Ld above this get no per-lane kernels: their per-lane
@@ -266,56 +285,135 @@ LANE_LSUM_MAX = 5
 
 
 def has_lane_kernels(La, Lb, Lc, Ld):
-    return La + Lb + Lc + Ld <= LANE_LSUM_MAX
+    return La + Lb + Lc + Ld <= LANE_LSUM_MAX and max(La, Lb, Lc, Ld) <= 2
+
+
+# Per-workgroup LDS the wave-per-quartet kernel may use; classes whose node
+# programs need more keep their slots in a global scratch buffer instead.
+COOP_LDS_CAP = 64 * 1024
+COOP_NT = 64          # one wave per quartet
+COOP_GLOBAL_SLICES = 1024   # workgroups of a launch with global slots (qf_types.h)
+
+
+class SlotPool:
+    """Lowest-free-slot allocator for node-program values."""
+
+    def __init__(self, first):
+        self.next = first
+        self.free = []
+
+    def take(self):
+        import heapq
+        if self.free:
+            return heapq.heappop(self.free)
+        self.next += 1
+        return self.next - 1
+
+    def give(self, slot):
+        import heapq
+        heapq.heappush(self.free, slot)
+
+
+def _allocate(values, levels, consumers_last, pool, pinned=()):
+    """Assigns slots level by level: before the values of level l are
+    placed, every value whose last reader ran at a level < l gives its slot
+    back.  A slot is therefore never written in the level that reads its old
+    value.  Values without readers go to no slot (None)."""
+    slot = dict(pinned)
+    by_level = {}
+    for v in values:
+        by_level.setdefault(levels[v], []).append(v)
+    live = [(consumers_last[v], v) for v in slot if v in consumers_last]
+    for lv in sorted(by_level):
+        still = []
+        for last, v in live:
+            if last < lv:
+                pool.give(slot[v])
+            else:
+                still.append((last, v))
+        live = still
+        for v in by_level[lv]:
+            if v in slot:
+                continue
+            if consumers_last.get(v, -1) < 0:
+                slot[v] = None
+                continue
+            slot[v] = pool.take()
+            live.append((consumers_last[v], v))
+    return slot
 
 
 def coop_tables(La, Lb, Lc, Ld):
     """Node programs of the wave-per-quartet kernel for one class.
 
-    LDS slots (doubles): [0, NACC) contracted [e0|f0] accumulators; from NACC
-    a region shared by the VRR (base [00|00]^(m), m = 0..L, the VRR nodes in
-    level order, one zero slot) and afterwards the HRR (ket nodes, then bra
-    nodes, each in level order); then the coefficient table (per (bra|ket,
-    direction): PA|QC, WP|WQ, 1/2z|1/2n, rho/z|rho/n; 1/2(z+n); CD and AB),
-    the integrals with cons and degeneracy, and the six density blocks."""
+    Slots (doubles): [0, NACC) contracted [e0|f0] accumulators; ZS a zero,
+    SINK a write-only slot; [OUT, OUT + NI) the integrals (ab|cd) in output
+    order; from POOL the VRR values of one primitive quartet ([00|00]^(m)
+    and the recursion nodes) and afterwards the HRR intermediates, both
+    placed by liveness over the levels; then the coefficient table CF
+    (per (bra|ket, direction): PA|QC, WP|WQ, 1/2z|1/2n, rho/z|rho/n;
+    1/2(z+n); CD and AB) and the six density blocks DM.
+
+    A VRR node of level l (= |e| + |f|) reads values of levels l-1 and l-2
+    only and, when its (e, f, m = 0) is a target, also adds itself into that
+    accumulator.  An HRR node of level l reads level l-1 (ket levels |d|,
+    then bra levels Ld + |b|); the last level writes the integrals."""
     v = Vrr(La, Lb, Lc, Ld)
     L = v.L
     acc = {t: i for i, t in enumerate(v.targets)}
     nacc = len(v.targets)
+    ZS, SINK = nacc, nacc + 1
+    na, nb, nc, nd = (len(comps(x)) for x in (La, Lb, Lc, Ld))
+    ni = na * nb * nc * nd
+    OUT = nacc + 2
+    POOL = OUT + ni
+    # ---- VRR ---------------------------------------------------------------
     order = [k for k in v.order() if v.nodes[k][0] != "base"]
-    nv = len(order)
-    slot = {}
-    for m in range(L + 1):
-        slot[(Z, Z, m)] = nacc + m
-    for idx, k in enumerate(order):
-        slot[k] = nacc + L + 1 + idx
-    zs = nacc + L + 1 + nv
+    vlevel = {k: am(k[0]) + am(k[1]) for k in v.nodes}
+    last = {}
+    for k in order:
+        kind, (i, deps) = v.nodes[k]
+        for d in deps:
+            for x in d[2:]:
+                last[x] = max(last.get(x, -1), vlevel[k])
+    base = [(Z, Z, m) for m in range(L + 1)]
+    for b in base:                      # the setup writes every F_m(T): give each a slot
+        last.setdefault(b, 0)
+    pool = SlotPool(POOL)
+    vslot = _allocate(base + order, vlevel, last, pool)
+    vmax = pool.next
     vnodes, vlev = [], [0]
     cur = None
+    sl = lambda k: vslot[k] if vslot[k] is not None else ZS
     for idx, k in enumerate(order):
-        lv = am(k[0]) + am(k[1])
+        lv = vlevel[k]
         if cur is not None and lv != cur:
             vlev.append(idx)
         cur = lv
         kind, (i, deps) = v.nodes[k]
-        a, b = slot[deps[0][2]], slot[deps[1][2]]
-        c = d = e = zs
+        a, b = sl(deps[0][2]), sl(deps[1][2])
+        c = d = e = ZS
         n1 = n2 = 0
         if kind == "bra":
             sel = i
             if len(deps) > 2:
-                n1, c, d = deps[2][1], slot[deps[2][2]], slot[deps[2][3]]
+                n1, c, d = deps[2][1], sl(deps[2][2]), sl(deps[2][3])
         else:
             sel = 3 + i
             for dd in deps[2:]:
                 if dd[0] == "N2":
-                    n1, c, d = dd[1], slot[dd[2]], slot[dd[3]]
+                    n1, c, d = dd[1], sl(dd[2]), sl(dd[3])
                 else:
-                    n2, e = dd[1], slot[dd[2]]
-        vnodes.append((a, b, c, d, e, slot[k], sel, n1, n2))
-    vlev.append(nv)
-    tgt = [slot[(e, f, 0)] for (e, f) in v.targets]
-    # HRR node programs
+                    n2, e = dd[1], sl(dd[2])
+        dst = vslot[k] if vslot[k] is not None else SINK
+        ac = acc[(k[0], k[1])] if (k[2] == 0 and (k[0], k[1]) in acc) else 0xFFFF
+        assert n1 < 16 and n2 < 16
+        vnodes.append((a, b, c, d, e, dst, ac, sel | (n1 << 4) | (n2 << 8)))
+    vlev.append(len(order))
+    fslot = [vslot[b] for b in base]
+    # every target must be produced by exactly one node
+    assert sorted(x[6] for x in vnodes if x[6] != 0xFFFF) == list(range(nacc))
+    # ---- HRR ---------------------------------------------------------------
     h = Hrr(La, Lb, Lc, Ld)
     outs = []
     for a in comps(La):
@@ -324,96 +422,101 @@ def coop_tables(La, Lb, Lc, Ld):
                 for d in comps(Ld):
                     h.need_bra(a, b, c, d)
                     outs.append((a, b, c, d))
-    korder = sorted(h.order_k, key=lambda k: am(k[2]))
-    border = sorted(h.order_b, key=lambda k: am(k[1]))
-    nk, nb_ = len(korder), len(border)
-    kslot = {k: nacc + i for i, k in enumerate(korder)}
-    bslot = {k: nacc + nk + i for i, k in enumerate(border)}
+    oidx = {o: x for x, o in enumerate(outs)}
 
-    def ks(e, c, d):
-        return acc[(e, c)] if d == Z else kslot[(e, c, d)]
+    def kkey(e, c, d):
+        return ("acc", e, c) if d == Z else ("k", e, c, d)
 
-    def bs(a, b, c, d):
-        return ks(a, c, d) if b == Z else bslot[(a, b, c, d)]
+    def bkey(a, b, c, d):
+        return kkey(a, c, d) if b == Z else ("b", a, b, c, d)
 
+    hn = {}        # key -> (level, sel, src1 key, src2 key)
+    for (e, c, d) in h.order_k:
+        i = h.ket[(e, c, d)]
+        hn[("k", e, c, d)] = (am(d), i, kkey(e, add(c, i), sub(d, i)), kkey(e, c, sub(d, i)))
+    for (a, b, c, d) in h.order_b:
+        i = h.bra[(a, b, c, d)]
+        hn[("b", a, b, c, d)] = (Ld + am(b), 3 + i, bkey(add(a, i), sub(b, i), c, d), bkey(a, sub(b, i), c, d))
+    out_keys = {bkey(*o): oidx[o] for o in outs}
+    copies = [k for k in out_keys if k[0] == "acc"]       # Lb = Ld = 0: the accumulators are the integrals
+    for k in copies:
+        hn[("copy",) + k[1:]] = (1, 0, k, "ZERO")
+    hlevel = {k: lv for k, (lv, _, _, _) in hn.items()}
+    hlast = {}
+    for k, (lv, _, s1, s2) in hn.items():
+        for x in (s1, s2):
+            hlast[x] = max(hlast.get(x, -1), lv)
+    accs = [("acc", e, f) for (e, f) in v.targets]
+    for a_ in accs:
+        hlevel[a_] = 0
+    pinned = {("acc", e, f): acc[(e, f)] for (e, f) in v.targets}
+    pinned["ZERO"] = ZS
+    hlevel["ZERO"] = 0
+    for k in hn:
+        if k in out_keys:
+            pinned[k] = OUT + out_keys[k]
+    for k in copies:
+        pinned[("copy",) + k[1:]] = OUT + out_keys[k]
+    hpool = SlotPool(POOL)
+    inner = [k for k in hn if k not in pinned]
+    for k in inner:
+        hlast.setdefault(k, -1)
+    # accumulator slots go back to the pool once their last reader has run
+    hslot = _allocate(accs + ["ZERO"] + sorted(inner, key=lambda k: hlevel[k]) + [k for k in hn if k in pinned],
+                      hlevel, hlast, hpool, pinned=pinned)
+    hmax = hpool.next
+    hkeys = sorted(hn, key=lambda k: (hlevel[k], str(k)))
     hnodes, hlev = [], [0]
     cur = None
-    for idx, (e, c, d) in enumerate(korder):
-        lv = ("k", am(d))
+    for idx, k in enumerate(hkeys):
+        lv = hlevel[k]
         if cur is not None and lv != cur:
             hlev.append(idx)
         cur = lv
-        i = h.ket[(e, c, d)]
-        hnodes.append((kslot[(e, c, d)], ks(e, add(c, i), sub(d, i)), ks(e, c, sub(d, i)), i))
-    for idx, (a, b, c, d) in enumerate(border):
-        lv = ("b", am(b))
-        if cur is not None and lv != cur:
-            hlev.append(nk + idx)
-        cur = lv
-        i = h.bra[(a, b, c, d)]
-        hnodes.append((bslot[(a, b, c, d)], bs(add(a, i), sub(b, i), c, d), bs(a, sub(b, i), c, d), 3 + i))
-    hlev.append(nk + nb_)
-    omap = [bs(*o) for o in outs]
-    region = max(L + 2 + nv, nk + nb_)
-    na, nb, nc, nd = (len(comps(x)) for x in (La, Lb, Lc, Ld))
-    ni = na * nb * nc * nd
-    cf = nacc + region
-    ic = cf + 32
-    dm = ic + ni
+        _, sel, s1, s2 = hn[k]
+        assert hslot[k] is not None
+        hnodes.append((hslot[k], hslot[s1], hslot[s2], sel))
+    hlev.append(len(hkeys))
+    foot = max(vmax, hmax, POOL)
+    cf = foot
+    dm = cf + 32
     total = dm + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd
-    assert total * 8 <= 64 * 1024, (La, Lb, Lc, Ld, total)
-    assert all(max(x[:6]) < 65536 for x in vnodes) and all(max(x[:3]) < 65536 for x in hnodes)
-    return dict(L=L, nacc=nacc, nv=nv, vlev=vlev, vnodes=vnodes, tgt=tgt, zs=zs, nk=nk, nb=nb_, hnodes=hnodes,
-                hlev=hlev, omap=omap, ni=ni, cf=cf, ic=ic, dm=dm, total=total, outs=outs, targets=v.targets)
+    assert total < 65536
+    return dict(L=L, nacc=nacc, zs=ZS, sink=SINK, out=OUT, pool=POOL, nv=len(order), vlev=vlev, vnodes=vnodes,
+                fslot=fslot, nh=len(hnodes), hnodes=hnodes, hlev=hlev, ni=ni, cf=cf, dm=dm, total=total, outs=outs,
+                targets=v.targets, glob=total * 8 > COOP_LDS_CAP)
 
 
 def coop_source(name, La, Lb, Lc, Ld):
     t = coop_tables(La, Lb, Lc, Ld)
-    L, nacc, nv, vlev, vnodes, tgt, zs = t["L"], t["nacc"], t["nv"], t["vlev"], t["vnodes"], t["tgt"], t["zs"]
-    nk, nb_, hnodes, hlev, omap, ni = t["nk"], t["nb"], t["hnodes"], t["hlev"], t["omap"], t["ni"]
-    cf, ic, dm, total = t["cf"], t["ic"], t["dm"], t["total"]
     cname = name + "Coop"
     src = ["struct %s {" % cname,
-           "  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L),
+           "  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, t["L"]),
            "  static constexpr int NACC = %d, NV = %d, NLV = %d, NH = %d, NLH = %d, NI = %d;"
-           % (nacc, nv, len(vlev) - 1, nk + nb_, len(hlev) - 1, ni),
-           "  static constexpr int BASE = %d, ZS = %d, CF = %d, IC = %d, DM = %d, LDS = %d;"
-           % (nacc, zs, cf, ic, dm, total),
-           "  static constexpr int NT = %d;   // threads per workgroup" % (256 if nv >= 1000 else 64),
+           % (t["nacc"], t["nv"], len(t["vlev"]) - 1, t["nh"], len(t["hlev"]) - 1, t["ni"]),
+           "  static constexpr int ZS = %d, SINK = %d, OUT = %d, CF = %d, DM = %d, LDS = %d;"
+           % (t["zs"], t["sink"], t["out"], t["cf"], t["dm"], t["total"]),
+           "  static constexpr bool GLOBAL_SLOTS = %s;   // slots in a global scratch buffer (over %d KiB)"
+           % ("true" if t["glob"] else "false", COOP_LDS_CAP // 1024),
+           "  static constexpr int NT = %d;   // threads per workgroup: one wave" % COOP_NT,
+           "  static constexpr unsigned short FSLOT[%d] = {%s};   // slots of [00|00]^(m)"
+           % (t["L"] + 1, ",".join(map(str, t["fslot"]))),
            "  __device__ static const CoopVNode* vrr();",
            "  __device__ static const int* vlev();",
-           "  __device__ static const unsigned short* tgt();",
            "  __device__ static const CoopHNode* hrr();",
            "  __device__ static const int* hlev();",
-           "  __device__ static const unsigned short* out();",
            "};"]
-    fmt_v = ",".join("{%d,%d,%d,%d,%d,%d,%d,%d,%d,0}" % x for x in vnodes)
-    fmt_h = ",".join("{%d,%d,%d,%d,0}" % x for x in hnodes)
-    src += ["__device__ const CoopVNode k%s_vrr[%d] = {%s};" % (cname, nv, fmt_v),
-            "__device__ const int k%s_vlev[%d] = {%s};" % (cname, len(vlev), ",".join(map(str, vlev))),
-            "__device__ const unsigned short k%s_tgt[%d] = {%s};" % (cname, nacc, ",".join(map(str, tgt))),
-            "__device__ const CoopHNode k%s_hrr[%d] = {%s};" % (cname, max(1, nk + nb_), fmt_h or "{0,0,0,0,0}"),
-            "__device__ const int k%s_hlev[%d] = {%s};" % (cname, len(hlev), ",".join(map(str, hlev))),
-            "__device__ const unsigned short k%s_out[%d] = {%s};" % (cname, ni, ",".join(map(str, omap))),
+    fmt_v = ",".join("{%d,%d,%d,%d,%d,%d,%d,%d}" % x for x in t["vnodes"])
+    fmt_h = ",".join("{%d,%d,%d,%d}" % x for x in t["hnodes"])
+    src += ["__device__ const CoopVNode k%s_vrr[%d] = {%s};" % (cname, t["nv"], fmt_v),
+            "__device__ const int k%s_vlev[%d] = {%s};" % (cname, len(t["vlev"]), ",".join(map(str, t["vlev"]))),
+            "__device__ const CoopHNode k%s_hrr[%d] = {%s};" % (cname, max(1, t["nh"]), fmt_h or "{0,0,0,0}"),
+            "__device__ const int k%s_hlev[%d] = {%s};" % (cname, len(t["hlev"]), ",".join(map(str, t["hlev"]))),
             "__device__ inline const CoopVNode* %s::vrr() { return k%s_vrr; }" % (cname, cname),
             "__device__ inline const int* %s::vlev() { return k%s_vlev; }" % (cname, cname),
-            "__device__ inline const unsigned short* %s::tgt() { return k%s_tgt; }" % (cname, cname),
             "__device__ inline const CoopHNode* %s::hrr() { return k%s_hrr; }" % (cname, cname),
-            "__device__ inline const int* %s::hlev() { return k%s_hlev; }" % (cname, cname),
-            "__device__ inline const unsigned short* %s::out() { return k%s_out; }" % (cname, cname)]
-    return cname, "\n".join(src) + "\n", total
-
-
-PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
-
-
-def quartet_classes():
-    out = []
-    for kb, (La, Lb) in enumerate(PAIR_CLASSES):
-        for kk, (Lc, Ld) in enumerate(PAIR_CLASSES):
-            if kk <= kb:
-                out.append((La, Lb, Lc, Ld))
-    return out
+            "__device__ inline const int* %s::hlev() { return k%s_hlev; }" % (cname, cname)]
+    return cname, "\n".join(src) + "\n", t["total"]
 
 
 def min_waves(nacc, ni):
@@ -460,7 +563,7 @@ def gen_class(La, Lb, Lc, Ld):
     L = La + Lb + Lc + Ld
     src = []
     src.append("// (%s%s|%s%s): %d accumulators, %d integrals, VRR %d FLOP/prim, HRR %d FLOP"
-               % ("spd"[La], "spd"[Lb], "spd"[Lc], "spd"[Ld], nacc, ni, vflops, hflops))
+               % ("spdf"[La], "spdf"[Lb], "spdf"[Lc], "spdf"[Ld], nacc, ni, vflops, hflops))
     src.append("struct %s {" % name)
     src.append("  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L))
     src.append("  static constexpr int NACC = %d, NI = %d;" % (nacc, ni))
@@ -512,22 +615,28 @@ def launcher_source(name, diag, coop=False, lane=True):
                 " g, ng, t, b, m);" % name,
                 "}", ""]
     if coop:
-        out += ["void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int grid,"
-                " hipStream_t s)" % name, "{",
+        out += ["// gscratch: COOP_GLOBAL_SLICES slices of LDS doubles when the class keeps its",
+                "// slots in global memory (the grid is then capped to that many workgroups)",
+                "void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int grid,"
+                " double* gscratch, hipStream_t s)" % name, "{",
+                "    using D = %sCoop;" % name,
                 "    if (nt <= 0) return;",
                 "    if (grid <= 0 || grid > nt) grid = nt;",
-                "    hipLaunchKernelGGL((fock_coop_kernel<%sCoop>), dim3(grid), dim3(%sCoop::NT), %sCoop::LDS * sizeof(double),"
-                " s, t, nt, b, m);" % (name, name, name),
+                "    if (D::GLOBAL_SLOTS && grid > COOP_GLOBAL_SLICES) grid = COOP_GLOBAL_SLICES;",
+                "    hipLaunchKernelGGL((fock_coop_kernel<D>), dim3(grid), dim3(D::NT),"
+                " D::GLOBAL_SLOTS ? 0 : D::LDS * sizeof(double), s, t, nt, b, m, gscratch);",
                 "}", ""]
     if diag:
-        out += ["void launch_schwarz_%s(PairBlock blk, double* out, hipStream_t s)" % name, "{",
+        out += ["void launch_schwarz_%s(PairBlock blk, double* out, double* gscratch, hipStream_t s)" % name, "{",
                 "    if (blk.n <= 0) return;"]
         if lane:
             out += ["    hipLaunchKernelGGL((schwarz_kernel<%s>), dim3((blk.n + WAVE - 1) / WAVE), dim3(WAVE), 0, s,"
                     " blk, out);" % name]
         else:
-            out += ["    hipLaunchKernelGGL((schwarz_coop_kernel<%sCoop>), dim3(blk.n), dim3(%sCoop::NT),"
-                    " %sCoop::LDS * sizeof(double), s, blk, out);" % (name, name, name)]
+            out += ["    using D = %sCoop;" % name,
+                    "    const int grid = D::GLOBAL_SLOTS && blk.n > COOP_GLOBAL_SLICES ? COOP_GLOBAL_SLICES : blk.n;",
+                    "    hipLaunchKernelGGL((schwarz_coop_kernel<D>), dim3(grid), dim3(D::NT),"
+                    " D::GLOBAL_SLOTS ? 0 : D::LDS * sizeof(double), s, blk, out, gscratch);"]
         out += ["}", ""]
     out += ["}  // namespace qf", ""]
     return "\n".join(out)
@@ -551,7 +660,8 @@ def main():
         name, src, info = gen_class(La, Lb, Lc, Ld)
         hdr = ("// Generated by quick_amd/codegen/gen_eri.py - do not edit.\n"
                "#pragma once\n#include \"../qf_device.h\"\nnamespace qf {\n" + src + "}  // namespace qf\n")
-        _write(os.path.join(OUT, name + ".h"), hdr)
+        if has_lane_kernels(La, Lb, Lc, Ld):       # the per-lane recursions of the other classes are not built
+            _write(os.path.join(OUT, name + ".h"), hdr)
         diag = (La, Lb) == (Lc, Ld)
         coop = (La, Lb, Lc, Ld) in COOP_CLASSES
         if coop:
@@ -575,8 +685,9 @@ def main():
              "using FockLaunch = int (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
              "using FockTileLaunch = void (*)(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
              " int, int, size_t, int, hipStream_t);",
-             "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);",
-             "using SchwarzLaunch = void (*)(PairBlock, double*, hipStream_t);", ""]
+             "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, double*,"
+             " hipStream_t);",
+             "using SchwarzLaunch = void (*)(PairBlock, double*, double*, hipStream_t);", ""]
     for (name, La, Lb, Lc, Ld, diag) in names:
         if has_lane_kernels(La, Lb, Lc, Ld):
             lines.append("int launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
@@ -584,10 +695,10 @@ def main():
             lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
                          " int, int, size_t, int, hipStream_t);" % name)
         if (La, Lb, Lc, Ld) in COOP_CLASSES:
-            lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, hipStream_t);"
-                         % name)
+            lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, double*,"
+                         " hipStream_t);" % name)
         if diag:
-            lines.append("void launch_schwarz_%s(PairBlock, double*, hipStream_t);" % name)
+            lines.append("void launch_schwarz_%s(PairBlock, double*, double*, hipStream_t);" % name)
     # FLOP model (FMA = 2): per surviving primitive quartet the VRR plus
     # BOYS_SETUP_FLOP(L) for the Boys function and the quartet setup, per
     # surviving contracted quartet the HRR plus 12 per Cartesian integral
@@ -616,7 +727,15 @@ def main():
     for (name, La, Lb, Lc, Ld, diag) in names:
         if (La, Lb, Lc, Ld) in COOP_CLASSES:
             lines.append("    if (cb == %d && ck == %d) return launch_fock_coop_%s;" % (pc[(La, Lb)], pc[(Lc, Ld)], name))
-    lines += ["    return nullptr;", "}", "", "inline SchwarzLaunch schwarz_launcher(int c)", "{"]
+    lines += ["    return nullptr;", "}", "",
+              "// doubles per workgroup slice of the global slot buffer (0: the class uses LDS)",
+              "inline int coop_global_doubles(int cb, int ck)", "{"]
+    for (name, La, Lb, Lc, Ld, diag) in names:
+        if (La, Lb, Lc, Ld) in COOP_CLASSES:
+            t = coop_tables(La, Lb, Lc, Ld)
+            if t["glob"]:
+                lines.append("    if (cb == %d && ck == %d) return %d;" % (pc[(La, Lb)], pc[(Lc, Ld)], t["total"]))
+    lines += ["    return 0;", "}", "", "inline SchwarzLaunch schwarz_launcher(int c)", "{"]
     for (name, La, Lb, Lc, Ld, diag) in names:
         if diag:
             lines.append("    if (c == %d) return launch_schwarz_%s;" % (pc[(La, Lb)], name))

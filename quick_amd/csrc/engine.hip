@@ -48,7 +48,8 @@ const char* const kDirectFp32Refused =
     "the direct-atomic kernels run FP64 only (their FP32 form gave wrong Fock matrices on gfx950 and is not "
     "built); FP32 halves run on the LDS-tile kernel";
 constexpr int kCoopWindow = 8;   // kets per task of the wave-per-quartet kernels
-const int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+constexpr int kNPC = 10;   // pair classes (La >= Lb) for s, p, d, f
+const int kClassL[kNPC][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
 inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
 template <typename T>
@@ -234,7 +235,7 @@ struct qf_engine {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
         int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
     };
-    ClassTune ctune[6][6];
+    ClassTune ctune[kNPC][kNPC];
     // pair blocks
     std::vector<HostBlock> blocks;
     std::vector<qf::PairBlock> hblocks;
@@ -266,6 +267,7 @@ struct qf_engine {
     DevBuf<double> d_zero;                          // nbasis^2 zeros: Coulomb density of the exchange-only pass
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
+    DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
     qf::MolDesc mol[2]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2
     // per-launch timing (HIP events on the build stream)
     bool timing = false;
@@ -488,10 +490,15 @@ int compute_schwarz(qf_engine* e)
         for (auto& b : blocks) total += b.info.size();
         DevBuf<double> dout;
         QF_HIP(dout.resize(total));
+        // global slot buffer of the wave-per-quartet diagonals that need one
+        size_t gdoubles = 0;
+        for (auto& b : blocks) gdoubles = std::max<size_t>(gdoubles, (size_t)qf::coop_global_doubles(b.cls, b.cls));
+        DevBuf<double> dgs;
+        if (gdoubles) QF_HIP(dgs.resize(gdoubles * qf::COOP_GLOBAL_SLICES));
         size_t off = 0;
         for (size_t k = 0; k < blocks.size(); ++k) {
             qf::SchwarzLaunch L = qf::schwarz_launcher(blocks[k].cls);
-            L(hb[k], dout.p + off, 0);
+            L(hb[k], dout.p + off, dgs.p, 0);
             off += blocks[k].info.size();
         }
         QF_HIP(hipGetLastError());
@@ -637,7 +644,7 @@ int prepare(qf_engine* e)
                       : e->nbasis <= 128  ? 2
                       : e->nbasis <= 256  ? 4
                                           : 12;
-    for (int cb = 0; cb < 6; ++cb)
+    for (int cb = 0; cb < kNPC; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32) {
                 const int offset = (int)e->tasks.size();
@@ -841,6 +848,12 @@ int prepare(qf_engine* e)
     }
     QF_HIP(e->d_tasks.upload(e->tasks));
     QF_HIP(e->d_groups.upload(e->groups));
+    // global slot buffer of the wave-per-quartet classes that keep their
+    // node-program slots in global memory
+    size_t gdoubles = 0;
+    for (const auto& L : e->launches)
+        if (L.tile == 2) gdoubles = std::max<size_t>(gdoubles, (size_t)qf::coop_global_doubles(L.cb, L.ck));
+    if (gdoubles) QF_HIP(e->d_cscratch.resize(gdoubles * qf::COOP_GLOBAL_SLICES));
 
     // ---- per-build buffers -------------------------------------------------
     const size_t n2 = (size_t)e->nbasis * e->nbasis;
@@ -857,7 +870,7 @@ int prepare(qf_engine* e)
     }
     QF_HIP(e->d_f0.upload(f0));
     QF_HIP(e->d_f1.upload(f1));
-    QF_HIP(e->d_mol.resize(2));
+    QF_HIP(e->d_mol.resize(3));   // slots: build / open-shell pass 1, open-shell pass 2, no-alloc build
     QF_HIP(e->d_counts.resize(2 * std::max<size_t>(1, e->launches.size())));
     e->stats.device_bytes = (long long)(e->d_info.bytes() + e->d_prim.bytes() + e->d_tasks.bytes() +
                                         e->d_Gt.bytes() * 3 + e->d_cm.bytes() + e->d_blocks.bytes());
@@ -874,30 +887,40 @@ int prepare(qf_engine* e)
 // One pass over every class launch: G(P) with the density screen dcm,
 // Coulomb terms on or off, then F += G and F2 += sign2 G.  `slot` picks the
 // device MolDesc, so two passes can be queued back to back on one stream.
+// Where a build keeps G: `replicas` copies of nbasis^2 at Gt (the atomics'
+// targets) and the symmetrised G at Gs.  The engine's own buffers, or the
+// fixed scratch array of the no-alloc entry point.
+struct GBufs {
+    double* Gt;
+    int replicas;
+    double* Gs;
+    bool timing;   // per-launch HIP events (never in the no-alloc path: it creates events)
+};
+
 int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, double* dF, double* dF2, double sign2,
-               int slot, hipStream_t s)
+               int slot, hipStream_t s, const GBufs& gb)
 {
     const int n = e->nbasis, ns = e->nshell;
     qf::MolDesc md{};
     md.P = dP;
-    md.G = e->d_Gt.p;
+    md.G = gb.Gt;
     md.cutmat = dcm;
     md.nbasis = n;
     md.nshell = ns;
     md.cutoff = e->cutoff;
     md.primlim = e->primlim;
     md.hyb = e->hyb;
-    md.replicas = e->replicas;
+    md.replicas = gb.replicas;
     md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
     md.PJ = jon ? dP : e->d_zero.p;
     e->mol[slot] = md;
     qf::MolDesc* dmol = e->d_mol.p + slot;
     QF_HIP(hipMemcpyAsync(dmol, &e->mol[slot], sizeof(md), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, e->d_Gt.p, (size_t)n * n * e->replicas);
+    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, gb.Gt, (size_t)n * n * gb.replicas);
     for (size_t li = 0; li < e->launches.size(); ++li) {
         const auto& L = e->launches[li];
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (e->timing) {
+        if (gb.timing) {
             while (e->ev_pool.size() < e->ev_used + 2) {
                 hipEvent_t ev;
                 QF_HIP(hipEventCreate(&ev));
@@ -912,7 +935,7 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
         if (L.tile == 2) {
             qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
             if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, e->persist, s);
+            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, e->persist, e->d_cscratch.p, s);
         } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
@@ -924,9 +947,9 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
             if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, e->persist, s))
                 return fail(QF_ERR_ARG, kDirectFp32Refused);
         }
-        if (e->timing) QF_HIP(hipEventRecord(eb, s));
+        if (gb.timing) QF_HIP(hipEventRecord(eb, s));
     }
-    hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, e->d_Gt.p, e->replicas, e->d_Gs.p,
+    hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, gb.Gt, gb.replicas, gb.Gs,
                        dF, dF2, sign2, n);
     QF_HIP(hipGetLastError());
     return QF_OK;
@@ -942,7 +965,7 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
                            e->d_f1.p, e->d_cm.p);
         dcm = e->d_cm.p;
     }
-    const int rc = build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s);
+    const int rc = build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s, GBufs{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing});
     if (rc == QF_OK) {
         e->built = true;
         e->count_cm_own = (dcm == e->d_cm.p);
@@ -975,9 +998,10 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
                            e->d_f1.p, e->d_cm.p);
         dcm = e->d_cm.p;
     }
-    int rc = build_pass(e, e->d_Pt.p, dcm, 1, dFa, dFb, 1.0, 0, s);
+    const GBufs gb{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing};
+    int rc = build_pass(e, e->d_Pt.p, dcm, 1, dFa, dFb, 1.0, 0, s, gb);
     if (rc) return rc;
-    rc = build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s);
+    rc = build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s, gb);
     if (rc == QF_OK) {
         e->built = true;
         e->count_cm_own = (dcm == e->d_cm.p);
@@ -1032,8 +1056,8 @@ int qf_set_basis(qf_engine* e, int nshell, const int* shell_atom, const int* she
 {
     if (!e || nshell <= 0 || nprim_total <= 0 || nbasis <= 0) return fail(QF_ERR_ARG, "bad basis sizes");
     for (int s = 0; s < nshell; ++s) {
-        if (shell_lmin[s] < 0 || shell_lmax[s] > 2 || shell_lmin[s] > shell_lmax[s])
-            return fail(QF_ERR_ARG, "angular momentum above d is not supported");
+        if (shell_lmin[s] < 0 || shell_lmax[s] > 3 || shell_lmin[s] > shell_lmax[s])
+            return fail(QF_ERR_ARG, "angular momentum above f is not supported");
         if (shell_atom[s] < 0 || shell_atom[s] >= e->natom) return fail(QF_ERR_ARG, "shell atom out of range");
         if (shell_pstart[s] < 0 || shell_pstart[s] + shell_nprim[s] > nprim_total || shell_nprim[s] <= 0)
             return fail(QF_ERR_ARG, "shell primitive range out of range");
@@ -1304,7 +1328,7 @@ int qf_set_tile(qf_engine* e, int enable, int waves, int tasks_per_group, int ld
 int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int tasks_per_group, int bras_per_task)
 {
     if (!e) return fail(QF_ERR_ARG, "NULL");
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
     if (waves > 4 || bras_per_task > 64) return fail(QF_ERR_ARG, "waves <= 4 and bras_per_task <= 64");
     qf_engine::ClassTune& t = e->ctune[cb][ck];
     if (tile > 2) return fail(QF_ERR_ARG, "tile must be -1, 0 (direct), 1 (LDS tile) or 2 (wave per quartet)");
@@ -1319,7 +1343,7 @@ int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int t
 int qf_set_class_piece(qf_engine* e, int cb, int ck, int piece)
 {
     if (!e) return fail(QF_ERR_ARG, "NULL");
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
     e->ctune[cb][ck].piece = piece;
     e->prepared = false;
     return QF_OK;
@@ -1442,6 +1466,7 @@ struct qf_batch {
     DevBuf<qf::TileGroup> d_groups;
     DevBuf<qf::MolDesc> d_moldesc;
     DevBuf<double> d_P, d_F, d_Gt, d_Gs, d_cm;
+    DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
     size_t gt_total = 0;
     int max_n2 = 0, max_ns2 = 0;
     // graph replay
@@ -1488,7 +1513,7 @@ int batch_issue(qf_batch* b, hipStream_t s)
         if (L.tile == 2) {
             qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
             if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, b->persist, s);
+            fc(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, b->persist, b->d_cscratch.p, s);
         } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
@@ -1601,7 +1626,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
     auto cost = [](const qf::Task& x) {
         return (long long)(x.ke - x.kb) * x.ket_count * (x.pe - x.pb) * (x.qe - x.qb);
     };
-    for (int cb = 0; cb < 6; ++cb)
+    for (int cb = 0; cb < kNPC; ++cb)
         for (int ck = 0; ck <= cb; ++ck)
             for (int fp32 = 0; fp32 < 2; ++fp32)
                 for (int tile = 0; tile < 3; ++tile) {
@@ -1654,6 +1679,11 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
                     }
                     b->launches.push_back(C);
                 }
+    size_t gdoubles = 0;
+    for (const auto& L : b->launches)
+        if (L.tile == 2) gdoubles = std::max<size_t>(gdoubles, (size_t)qf::coop_global_doubles(L.cb, L.ck));
+    if (gdoubles && b->d_cscratch.resize(gdoubles * qf::COOP_GLOBAL_SLICES) != hipSuccess)
+        return bail(fail(QF_ERR_MEMORY, "batch global slot buffer"));
     if (b->d_blocks.upload(blocks) != hipSuccess || b->d_tasks.upload(tasks) != hipSuccess ||
         b->d_groups.upload(groups) != hipSuccess || b->d_moldesc.upload(md) != hipSuccess ||
         b->d_bm.upload(b->mols) != hipSuccess)

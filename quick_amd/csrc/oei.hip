@@ -1,6 +1,7 @@
 // One-electron matrices S, T, V for the SCF harness around the Fock build
 // (QUICK: quick_overlap_module.f90:172-363, quick_oei_module.f90:69-257,
-// attrashell.fh).  One thread per basis-function pair, McMurchie-Davidson
+// attrashell.fh).  One thread per basis-function pair (s through f),
+// McMurchie-Davidson
 // Hermite expansion; the nuclear-attraction Boys function is QUICK's FmT
 // called with Maxm = Qfinal(I) + Qfinal(J) as attrashell.fh:46-48,89.
 // Runs once per geometry, so it favours simplicity over speed.
@@ -21,13 +22,14 @@ struct OeiFn {
     double X, Y, Z;
 };
 
-// Hermite coefficients E^{i,j}_t (i <= 2, j <= 4, t <= 6) in one direction,
+// Hermite coefficients E^{i,j}_t (i <= 3, j <= 5, t <= 8) in one direction,
 // without the exp(-mu X_AB^2) factor.
-__device__ void hermite_table(int la, int lb, double XPA, double XPB, double oo2p, double (&E)[3][5][7])
+constexpr int kEI = 4, kEJ = 6, kET = 9;
+__device__ void hermite_table(int la, int lb, double XPA, double XPB, double oo2p, double (&E)[kEI][kEJ][kET])
 {
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 5; ++j)
-            for (int t = 0; t < 7; ++t) E[i][j][t] = 0.0;
+    for (int i = 0; i < kEI; ++i)
+        for (int j = 0; j < kEJ; ++j)
+            for (int t = 0; t < kET; ++t) E[i][j][t] = 0.0;
     E[0][0][0] = 1.0;
     for (int i = 0; i <= la; ++i)
         for (int j = 0; j <= lb; ++j) {
@@ -35,10 +37,10 @@ __device__ void hermite_table(int la, int lb, double XPA, double XPB, double oo2
             for (int t = 0; t <= i + j; ++t) {
                 double v;
                 if (i > 0) {
-                    v = XPA * E[i - 1][j][t] + (t + 1 <= 6 ? (t + 1) * E[i - 1][j][t + 1] : 0.0);
+                    v = XPA * E[i - 1][j][t] + (t + 1 < kET ? (t + 1) * E[i - 1][j][t + 1] : 0.0);
                     if (t > 0) v += oo2p * E[i - 1][j][t - 1];
                 } else {
-                    v = XPB * E[i][j - 1][t] + (t + 1 <= 6 ? (t + 1) * E[i][j - 1][t + 1] : 0.0);
+                    v = XPB * E[i][j - 1][t] + (t + 1 < kET ? (t + 1) * E[i][j - 1][t + 1] : 0.0);
                     if (t > 0) v += oo2p * E[i][j - 1][t - 1];
                 }
                 E[i][j][t] = v;
@@ -46,18 +48,30 @@ __device__ void hermite_table(int la, int lb, double XPA, double XPB, double oo2
         }
 }
 
-__device__ void boys_rt(double X, int maxm, int L, double (&F)[5])
+constexpr int kLV = 6;   // highest La + Lb of the nuclear attraction (f f)
+
+template <int L>
+__device__ void boys_copy(double X, int maxm, double (&G)[kLV + 1])
 {
-    // same algorithm as qf::boys_quick, with a run-time order (L <= 4)
-    double G[5];
+    double H[L + 1];
+    qf::boys_quick<L>(X, maxm, H, qf::kBoysFit);
+    for (int m = 0; m <= L; ++m) G[m] = H[m];
+}
+
+__device__ void boys_rt(double X, int maxm, int L, double (&F)[kLV + 1])
+{
+    // same algorithm as qf::boys_quick, with a run-time order (L <= 6)
+    double G[kLV + 1] = {};
     switch (L) {
-        case 0: { double H[1]; qf::boys_quick<0>(X, maxm, H, qf::kBoysFit); G[0] = H[0]; break; }
-        case 1: { double H[2]; qf::boys_quick<1>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 2; ++m) G[m] = H[m]; break; }
-        case 2: { double H[3]; qf::boys_quick<2>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 3; ++m) G[m] = H[m]; break; }
-        case 3: { double H[4]; qf::boys_quick<3>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 4; ++m) G[m] = H[m]; break; }
-        default: { double H[5]; qf::boys_quick<4>(X, maxm, H, qf::kBoysFit); for (int m = 0; m < 5; ++m) G[m] = H[m]; break; }
+        case 0: boys_copy<0>(X, maxm, G); break;
+        case 1: boys_copy<1>(X, maxm, G); break;
+        case 2: boys_copy<2>(X, maxm, G); break;
+        case 3: boys_copy<3>(X, maxm, G); break;
+        case 4: boys_copy<4>(X, maxm, G); break;
+        case 5: boys_copy<5>(X, maxm, G); break;
+        default: boys_copy<6>(X, maxm, G); break;
     }
-    for (int m = 0; m < 5; ++m) F[m] = m <= L ? G[m] : 0.0;
+    for (int m = 0; m <= kLV; ++m) F[m] = m <= L ? G[m] : 0.0;
 }
 
 __global__ void k_oei(const OeiFn* __restrict__ fn, int n, const double* __restrict__ exps,
@@ -83,7 +97,7 @@ __global__ void k_oei(const OeiFn* __restrict__ fn, int n, const double* __restr
             const double cb = coef[(b.pstart + j) * 4 + b.L] * b.cons;
             const double p = al + be, oo2p = 0.5 / p;
             const double Kab = exp(-al * be / p * AB2);
-            double P[3], E[3][3][5][7];
+            double P[3], E[3][kEI][kEJ][kET];
             for (int d = 0; d < 3; ++d) {
                 P[d] = (al * A[d] + be * B[d]) / p;
                 hermite_table(la[d], lb[d] + 2, P[d] - A[d], P[d] - B[d], oo2p, E[d]);
@@ -103,10 +117,10 @@ __global__ void k_oei(const OeiFn* __restrict__ fn, int n, const double* __restr
             for (int c = 0; c < natom; ++c) {
                 const double PC[3] = {P[0] - xyz[3 * c], P[1] - xyz[3 * c + 1], P[2] - xyz[3 * c + 2]};
                 const double Tb = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                double F[5];
+                double F[kLV + 1];
                 boys_rt(Tb, maxm, L, F);
                 // R^N_tuv level by level, N = L..0
-                double R[5][5][5], Rn[5][5][5];
+                double R[kLV + 1][kLV + 1][kLV + 1], Rn[kLV + 1][kLV + 1][kLV + 1];
                 for (int N = L; N >= 0; --N) {
                     double f = F[N];
                     for (int k = 0; k < N; ++k) f *= -2.0 * p;
@@ -160,11 +174,14 @@ extern "C" int qf_one_electron_ext(qf_engine* e, int ncenter, const double* cent
                                &nprim, &exps, &coef, &nbasis);
     if (rc) return rc;
     if (ncenter < 0 || (ncenter > 0 && (!center_xyz || !charges)) || !S || !T || !V) return QF_ERR_ARG;
-    static const int comp[3][6][3] = {{{0, 0, 0}},
-                                      {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
-                                      {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}}};
-    const double s3 = std::sqrt(3.0);
-    const double cons[3][6] = {{1}, {1, 1, 1}, {1, s3, 1, s3, s3, 1}};
+    // QUICK's component order and cons per shell type (basis.f90:567-720)
+    static const int comp[4][10][3] = {
+        {{0, 0, 0}},
+        {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+        {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}},
+        {{3, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 3, 0}, {2, 0, 1}, {1, 1, 1}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {0, 0, 3}}};
+    const double s3 = std::sqrt(3.0), s5 = std::sqrt(5.0), s15 = std::sqrt(15.0);
+    const double cons[4][10] = {{1}, {1, 1, 1}, {1, s3, 1, s3, s3, 1}, {1, s5, s5, 1, s5, s15, s5, s5, s5, 1}};
     std::vector<OeiFn> fn(nbasis);
     for (int s = 0; s < nshell; ++s) {
         int f = s_fstart[s];

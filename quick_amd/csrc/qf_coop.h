@@ -3,53 +3,61 @@
 // The per-lane kernels give every lane its own quartet, so a (dd|dd)
 // quartet's 961 accumulators and ~2,200 VRR intermediates live in scratch
 // memory and the launch is latency-bound at one wave per SIMD.  Here the 64
-// lanes of a wave share ONE quartet: the generated node programs (see
-// gen_eri.coop_tables) evaluate the VRR level by level - every node of a
-// level is independent, one node per lane - and then the HRR the same way,
-// with all intermediates in LDS.  Digestion is lane-parallel too: each lane
-// owns output elements of J_ab, J_cd, K_ac, K_ad, K_bc, K_bd and sums its
-// contraction over the integral block in LDS.
+// lanes of one wave (the whole workgroup) share ONE quartet: the generated
+// node programs (gen_eri.coop_tables) evaluate the VRR level by level - every
+// node of a level is independent, one node per lane - and then the HRR the
+// same way, with all values in LDS slots that the generator reuses by
+// liveness, so several workgroups fit on one CU.  A VRR node whose value is a
+// target also adds itself into its contracted accumulator.  Digestion is
+// lane-parallel too: each lane owns output elements of J_ab, J_cd, K_ac,
+// K_ad, K_bc, K_bd and sums its contraction over the integral block.
+//
+// Classes whose slots exceed the LDS cap keep them in a global scratch
+// buffer instead (one slice per workgroup, GLOBAL_SLOTS): the same code with
+// global instead of LDS addresses.
 //
 // The arithmetic is the same as in fock_task (same screening, same Boys
 // function, same primitive loops and prefactors, same digestion weights);
 // only the evaluation order of the recursions differs.
 #pragma once
+#include <type_traits>
+
 #include "qf_device.h"
 
 namespace qf {
 
 struct CoopVNode {
-    // value = C1 s[a] + C2 s[b] + n1 H (s[c] - Rr s[d]) + n2 (1/2(z+n)) s[e], written to s[dst];
+    // value = C1 s[a] + C2 s[b] + n1 H (s[c] - Rr s[d]) + n2 (1/2(z+n)) s[e], written to s[dst]
+    // (and added to s[acc] unless acc == 0xFFFF); meta = sel | n1 << 4 | n2 << 8;
     // (C1, C2, H, Rr) = (PA, WP, 1/2z, rho/z) for sel = dir (bra step) and
     // (QC, WQ, 1/2n, rho/n) for sel = 3 + dir (ket step)
-    unsigned short a, b, c, d, e, dst;
-    unsigned char sel, n1, n2, pad;
+    unsigned short a, b, c, d, e, dst, acc, meta;
 };
 
 struct CoopHNode {
     // s[dst] = s[s1] + X s[s2], X = CD[dir] (sel = dir) or AB[dir] (sel = 3 + dir)
-    unsigned short dst, s1, s2;
-    unsigned char sel, pad;
+    unsigned short dst, s1, s2, sel;
 };
 
-constexpr int kVU = 2;   // VRR nodes per thread in flight
-constexpr int kHU = 4;   // HRR nodes per thread in flight
+constexpr int kVU = 2;   // VRR nodes per lane in flight
+constexpr int kHU = 4;   // HRR nodes per lane in flight
 
+// Slot memory of a class: LDS, or a global scratch slice.
+template <class D>
+using CoopSlots = typename std::conditional<D::GLOBAL_SLOTS, QF_AS_GLOBAL double*, QF_AS_LDS double*>::type;
 
 // Contracted [e0|f0] accumulators of one quartet into s[0, NACC): the
 // primitive loops with QUICK's screen and order, as in contract().  The
 // bra/ket primitive-pair ranges [pb, pe), [qb, qe) select a piece of the
 // quartet.  With primlim < 0 nothing is screened (Schwarz bounds).
-template <class D>
+template <class D, typename SP>
 __device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, const PairBlock& bb, int i,
                                               const PairInfo& bi, const PairBlock& kb, int j, const PairInfo& ki,
-                                              double dn, double primlim, QF_AS_LDS double* s,
-                                              const QF_AS_LDS double* boys_tab)
+                                              double dn, double primlim, SP s, const QF_AS_LDS double* boys_tab)
 {
-    const int lane = threadIdx.x;   // thread index in the workgroup
-    QF_AS_LDS double* cf = s + D::CF;   // [0,24) VRR coefficients, 24: 1/2(z+n), [25,31): CD, AB
+    const int lane = threadIdx.x;
+    SP cf = s + D::CF;   // [0,24) VRR coefficients, 24: 1/2(z+n), [25,31): CD, AB
     for (int x = lane; x < D::NACC; x += D::NT) s[x] = 0.0;
-    __syncthreads();
 
     const auto bp = cptr(bb.prim);
     const auto kp = cptr(kb.prim);
@@ -58,7 +66,6 @@ __device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, co
     const double c0 = dn * bp[(PF_CUT * bK) * bn + i];
     const auto vrr = gptr(D::vrr());
     const auto vlev = cptr(D::vlev());
-    const auto tgt = gptr(D::tgt());
     for (int jp = qb; jp < qe; ++jp) {
         const double cutk = kp[(PF_CUT * kK + jp) * kn + j];
         if (!(c0 * cutk > primlim)) break;
@@ -84,13 +91,14 @@ __device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, co
             boys_quick<D::L>(T, maxm, F, boys_tab);
             const double pref = Kb * Kk * rs;
             const double wp = zk * inv, wq = zb * inv;
+            __syncthreads();   // the previous primitive's last level has read its sources
 #pragma unroll
             for (int m = 0; m <= D::L; ++m)
-                if (lane == m) s[D::BASE + m] = F[m] * pref;
+                if (lane == m) s[D::FSLOT[m]] = F[m] * pref;
             if (lane == 0) {
-                // the values are the same for the whole workgroup (one quartet):
-                // one lane stores them with compile-time indices (a lane-indexed
-                // read would place the pair records and these arrays in scratch)
+                // the values are the same for the whole wave (one quartet): one
+                // lane stores them with compile-time indices (a lane-indexed read
+                // would place the pair records and these arrays in scratch)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     cf[4 * d + 0] = Pb[d] - bi.A[d];
@@ -121,36 +129,39 @@ __device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, co
                     double v[kVU];
 #pragma unroll
                     for (int u = 0; u < kVU; ++u) {
-                        const QF_AS_LDS double* c4 = cf + 4 * q[u].sel;
-                        v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] +
-                               double(q[u].n1) * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
-                               double(q[u].n2) * cf[24] * s[q[u].e];
+                        const SP c4 = cf + 4 * (q[u].meta & 15);
+                        const double n1 = double((q[u].meta >> 4) & 15), n2 = double(q[u].meta >> 8);
+                        v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] + n1 * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
+                               n2 * cf[24] * s[q[u].e];
                     }
 #pragma unroll
                     for (int u = 0; u < kVU; ++u)
-                        if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+                        if (x0 + u * D::NT < e1) {
+                            s[q[u].dst] = v[u];
+                            if (q[u].acc != 0xFFFF) s[q[u].acc] += v[u];
+                        }
                 }
                 __syncthreads();
             }
-            for (int x = lane; x < D::NACC; x += D::NT) s[x] += s[tgt[x]];
-            __syncthreads();
         }
     }
 }
 
-// HRR on the contracted block in s (the VRR region is dead now); the
-// integral of output index x then sits at s[D::out()[x]].
-template <class D>
-__device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki, QF_AS_LDS double* s)
+// HRR on the contracted block in s (the VRR values are dead now); the
+// integral of output index x then sits at s[D::OUT + x].
+template <class D, typename SP>
+__device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki, SP s)
 {
     const int lane = threadIdx.x;
-    QF_AS_LDS double* cf = s + D::CF;
+    SP cf = s + D::CF;
+    __syncthreads();
     if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             cf[25 + d] = ki.AB[d];
             cf[28 + d] = bi.AB[d];
         }
+        s[D::ZS] = 0.0;
     }
     __syncthreads();
     const auto hrr = gptr(D::hrr());
@@ -175,34 +186,33 @@ __device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki,
     }
 }
 
-template <class D>
+template <class D, typename SP>
 __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb, int i, const PairInfo& bi,
                                              const PairBlock& kb, int j, const PairInfo& ki, double dn,
-                                             const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_LDS double* s,
+                                             const MolDesc& md, QF_AS_GLOBAL double* G, SP s,
                                              const QF_AS_LDS double* boys_tab)
 {
     constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
-    const int lane = threadIdx.x;   // thread index in the workgroup
+    const int lane = threadIdx.x;
     coop_contract<D>(t.pb, t.pe, t.qb, t.qe, bb, i, bi, kb, j, ki, dn, md.primlim, s, boys_tab);
     coop_hrr<D>(bi, ki, s);
 
-    // integrals with cons and degeneracy weight; density blocks
+    // integrals with cons and degeneracy weight, in place; density blocks
     const double sdeg = 0.25 * (bi.qa != bi.qb ? 2.0 : 1.0) * (ki.qa != ki.qb ? 2.0 : 1.0) * (bi.id != ki.id ? 2.0 : 1.0);
-    const auto out = gptr(D::out());
-    QF_AS_LDS double* Ic = s + D::IC;
+    SP Ic = s + D::OUT;
     for (int x = lane; x < D::NI; x += D::NT) {
         const int d = x % nd, c = (x / nd) % nc, b = (x / (nd * nc)) % nb, a = x / (nd * nc * nb);
         const double cons = kCons[D::La][a] * kCons[D::Lb][b] * kCons[D::Lc][c] * kCons[D::Ld][d];
-        Ic[x] = s[out[x]] * (cons * sdeg);
+        Ic[x] = Ic[x] * (cons * sdeg);
     }
     const int n = md.nbasis;
     const auto P = gptr(md.P);
-    QF_AS_LDS double* Dab = s + D::DM;
-    QF_AS_LDS double* Dcd = Dab + na * nb;
-    QF_AS_LDS double* Dac = Dcd + nc * nd;
-    QF_AS_LDS double* Dad = Dac + na * nc;
-    QF_AS_LDS double* Dbc = Dad + na * nd;
-    QF_AS_LDS double* Dbd = Dbc + nb * nc;
+    SP Dab = s + D::DM;
+    SP Dcd = Dab + na * nb;
+    SP Dac = Dcd + nc * nd;
+    SP Dad = Dac + na * nc;
+    SP Dbc = Dad + na * nd;
+    SP Dbd = Dbc + nb * nc;
     const auto PJ = gptr(md.PJ);              // Coulomb density (zeros in an exchange-only build)
     for (int x = lane; x < na * nb; x += D::NT) Dab[x] = PJ[(bi.fa + x / nb) * n + bi.fb + x % nb];
     for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = PJ[(ki.fa + x / nd) * n + ki.fb + x % nd];
@@ -264,23 +274,35 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
         if (v != 0.0) atomic_add(&G[row * n + col], v);
     }
 #undef IX
-    __syncthreads();   // the next quartet reuses the LDS
+    __syncthreads();   // the next quartet reuses the slots
 }
 
-// One workgroup of D::NT threads per task (tasks of narrow ket windows):
-// 4 waves for the largest node programs, one wave otherwise.  Every wave
-// screens the task's quartets lane-parallel (the same tests, so the same
-// ballot in each wave), then the workgroup processes the surviving quartets
-// one after another.  FP64 throughout.  Dynamic LDS: D::LDS doubles.
+// Slot memory of this workgroup: dynamic LDS, or its slice of the global
+// scratch buffer (gscratch holds gridDim.x slices of D::LDS doubles).
+template <class D>
+__device__ __forceinline__ CoopSlots<D> coop_slots(double* gscratch)
+{
+    if constexpr (D::GLOBAL_SLOTS) {
+        return (QF_AS_GLOBAL double*)(gscratch + (size_t)blockIdx.x * D::LDS);
+    } else {
+        extern __shared__ double qf_lds[];
+        return (QF_AS_LDS double*)qf_lds;
+    }
+}
+
+// One workgroup (one wave) per task of a narrow ket window: the wave
+// screens the task's quartets lane-parallel, then processes the surviving
+// quartets one after another.  FP64 throughout.  Dynamic LDS: D::LDS doubles
+// (none with GLOBAL_SLOTS).
 template <class D>
 __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict__ tasks, int ntasks,
                                                          const PairBlock* __restrict__ blocks,
-                                                         const MolDesc* __restrict__ mols)
+                                                         const MolDesc* __restrict__ mols, double* gscratch)
 {
-    extern __shared__ double qf_lds[];
+    static_assert(D::NT == WAVE, "one wave per workgroup");
     QF_BOYS_LDS_DECL;
     QF_BOYS_LDS_FILL();
-    QF_AS_LDS double* s = (QF_AS_LDS double*)qf_lds;
+    const auto s = coop_slots<D>(gscratch);
     const int lane = threadIdx.x % WAVE;
     for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
         const Task t = cptr(tasks)[ti];
@@ -319,38 +341,33 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
 }
 
 // Schwarz diagonal of a class (La Lb | La Lb) on the wave-per-quartet node
-// programs: one workgroup per pair of blk, out[j] = max over components
-// |(ab|ab)| with no screening (shellcutoff / classcutoff,
-// quick_cutoff_module.f90:91-424), like schwarz_kernel for the classes that
-// have per-lane kernels.
+// programs: one workgroup per pair of blk (striding when the grid is capped),
+// out[j] = max over components |(ab|ab)| with no screening (shellcutoff /
+// classcutoff, quick_cutoff_module.f90:91-424), like schwarz_kernel for the
+// classes that have per-lane kernels.
 template <class D>
-__global__ __launch_bounds__(D::NT) void schwarz_coop_kernel(PairBlock blk, double* __restrict__ out)
+__global__ __launch_bounds__(D::NT) void schwarz_coop_kernel(PairBlock blk, double* __restrict__ out,
+                                                            double* gscratch)
 {
     constexpr int na = ncart(D::La), nb = ncart(D::Lb);
     static_assert(D::La == D::Lc && D::Lb == D::Ld, "diagonal class");
-    extern __shared__ double qf_lds[];
     QF_BOYS_LDS_DECL;
     QF_BOYS_LDS_FILL();
-    QF_AS_LDS double* s = (QF_AS_LDS double*)qf_lds;
-    const int j = blockIdx.x;
-    const PairInfo pi = cptr(blk.info)[j];
-    coop_contract<D>(0, blk.K, 0, blk.K, blk, j, pi, blk, j, pi, 1.0, -1.0, s, QF_BOYS_LDS_PTR);
-    coop_hrr<D>(pi, pi, s);
-    const auto omap = gptr(D::out());
-    double m = 0.0;
-    for (int x = threadIdx.x; x < na * nb; x += D::NT) {
-        const int a = x / nb, b = x % nb;
-        const double cons = kCons[D::La][a] * kCons[D::Lb][b];
-        m = fmax(m, fabs(s[omap[((a * nb + b) * na + a) * nb + b]] * cons * cons));
-    }
+    const auto s = coop_slots<D>(gscratch);
+    for (int j = blockIdx.x; j < blk.n; j += gridDim.x) {
+        const PairInfo pi = cptr(blk.info)[j];
+        coop_contract<D>(0, blk.K, 0, blk.K, blk, j, pi, blk, j, pi, 1.0, -1.0, s, QF_BOYS_LDS_PTR);
+        coop_hrr<D>(pi, pi, s);
+        double m = 0.0;
+        for (int x = threadIdx.x; x < na * nb; x += D::NT) {
+            const int a = x / nb, b = x % nb;
+            const double cons = kCons[D::La][a] * kCons[D::Lb][b];
+            m = fmax(m, fabs(s[D::OUT + ((a * nb + b) * na + a) * nb + b] * cons * cons));
+        }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, WAVE));
-    __syncthreads();   // every lane has read its integrals; reuse s for the reduction
-    if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < D::NT / WAVE; ++w) m = fmax(m, s[w]);
-        out[j] = m;
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, WAVE));
+        if (threadIdx.x == 0) out[j] = m;
+        __syncthreads();   // the next pair reuses the slots
     }
 }
 

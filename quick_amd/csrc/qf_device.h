@@ -151,11 +151,14 @@ __host__ __device__ __forceinline__ void boys_quick(const double X, const int ma
     }
 }
 
-// cons factors of QUICK's Cartesian components (basis.f90:567-640)
-__host__ __device__ constexpr double kCons[3][6] = {
-    {1.0, 0, 0, 0, 0, 0},
-    {1.0, 1.0, 1.0, 0, 0, 0},
-    {1.0, 1.7320508075688772935, 1.0, 1.7320508075688772935, 1.7320508075688772935, 1.0}};
+// cons factors of QUICK's Cartesian components (basis.f90:567-720): d
+// xy, xz, yz sqrt(3); f xxy, xyy, xxz, yyz, xzz, yzz sqrt(5), xyz sqrt(15)
+__host__ __device__ constexpr double kCons[4][10] = {
+    {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {1.0, 1.0, 1.0, 0, 0, 0, 0, 0, 0, 0},
+    {1.0, 1.7320508075688772935, 1.0, 1.7320508075688772935, 1.7320508075688772935, 1.0, 0, 0, 0, 0},
+    {1.0, 2.2360679774997896964, 2.2360679774997896964, 1.0, 2.2360679774997896964, 3.8729833462074168852,
+     2.2360679774997896964, 2.2360679774997896964, 2.2360679774997896964, 1.0}};
 
 constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 

@@ -12,6 +12,10 @@ namespace qf {
 
 constexpr int WAVE = 64;   // lanes per wavefront on CDNA
 
+// Workgroups of a wave-per-quartet launch whose class keeps its node-program
+// slots in global memory: the slot buffer holds this many slices.
+constexpr int COOP_GLOBAL_SLICES = 1024;
+
 enum PrimField { PF_ZETA = 0, PF_OOZ2, PF_PX, PF_PY, PF_PZ, PF_KC, PF_CUT, PF_N };
 
 struct PairInfo {

@@ -15,7 +15,7 @@ from .basis import BasisSet
 
 
 class FockEngine:
-    """G(P) = J(P) - hyb/2 K(P) over a Cartesian s/p/d basis on one GPU.
+    """G(P) = J(P) - hyb/2 K(P) over a Cartesian s/p/d/f basis on one GPU.
 
     Screening follows QUICK's CPU get2e: a shell quartet runs when
     Y_ij*Y_kl > cutoff and Y_ij*Y_kl*DNmax > cutoff, a primitive quartet when
@@ -140,7 +140,7 @@ class FockEngine:
         """Algorithmic FLOPs of a build with the last density (per-class table)."""
         return self.lib.qf_flops_estimate(self.h)
 
-    CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd"]
+    CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd", "fs", "fp", "fd", "ff"]
 
     def launches(self):
         """[(name, fp32, ntasks)] of the class launches a build issues."""

@@ -1,15 +1,15 @@
 """Convert Gaussian94-format basis files (Basis Set Exchange output, as shipped
 in QUICK's ``basis/*.BAS``) into the compact JSON files this package loads.
 
-Run once in a container that has the reference checkout:
+Run once in a container that has the reference checkout (a provenance
+recipe: nothing at run time reads the reference):
 
-    python quick_amd/basis_data/extract_bse.py /root/reference/basis
+    python tests/golden/extract_bse.py /root/reference/basis
 
 The JSON keeps the published exponents and *unnormalised* contraction
 coefficients exactly as printed (Fortran ``D`` exponents converted to ``E``);
-normalisation happens at load time (``quick_amd.basis``).  Only shell types up
-to D are kept for the elements listed in ``ELEMENTS``; shells above D are
-recorded so a loader can refuse them.
+normalisation happens at load time (``quick_amd.basis``).  Shells are kept as
+printed (S, SP, P, D, F) for the elements listed in ``ELEMENTS``.
 """
 import json
 import os
@@ -24,6 +24,8 @@ FILES = {
     "6-31G**": "6-31GDP.BAS",
     "cc-pVDZ": "CC-PVDZ.BAS",
     "def2-SVP": "DEF2-SVP.BAS",
+    "cc-pVTZ": "CC-PVTZ.BAS",
+    "def2-TZVP": "DEF2-TZVP.BAS",
 }
 
 ELEMENTS = ["H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne",
@@ -82,4 +84,5 @@ def main(src_dir, dst_dir):
 
 if __name__ == "__main__":
     src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/basis"
-    main(src, os.path.dirname(os.path.abspath(__file__)))
+    here = os.path.dirname(os.path.abspath(__file__))
+    main(src, os.path.join(os.path.dirname(os.path.dirname(here)), "quick_amd", "basis_data"))

@@ -15,9 +15,14 @@ CART = {
     0: [(0, 0, 0)],
     1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
     2: [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)],
+    3: [(3, 0, 0), (2, 1, 0), (1, 2, 0), (0, 3, 0), (2, 0, 1), (1, 1, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+        (0, 0, 3)],
 }
-CONS = {0: [1.0], 1: [1.0] * 3, 2: [1.0, math.sqrt(3), 1.0, math.sqrt(3), math.sqrt(3), 1.0]}
-NH = 5  # Hermite index range per direction (0..4)
+_S5 = math.sqrt(5)
+CONS = {0: [1.0], 1: [1.0] * 3, 2: [1.0, math.sqrt(3), 1.0, math.sqrt(3), math.sqrt(3), 1.0],
+        3: [1.0, _S5, _S5, 1.0, _S5, math.sqrt(15), _S5, _S5, _S5, 1.0]}
+NH = 7  # Hermite index range per direction (0..6)
+_IDX = np.array([(t, u, v) for t in range(NH) for u in range(NH) for v in range(NH)])
 
 
 def hermite_E(la, lb, a, b, XA, XB):
@@ -120,10 +125,8 @@ def shell_block(sh, boys):
                                         Fx[0][ci[0], dj[0], t] * Fx[1][ci[1], dj[1], u] * Fx[2][ci[2], dj[2], v]
                     alpha = p * q / (p + q)
                     R = hermite_R(L, alpha, P - Q, boys)
-                    Rm = np.zeros((NH ** 3, NH ** 3))
-                    for k1, (t1, u1, v1) in enumerate(idx):
-                        for k2, (t2, u2, v2) in enumerate(idx):
-                            Rm[k1, k2] = R[t1 + t2, u1 + u2, v1 + v2]
+                    Rm = R[_IDX[:, None, 0] + _IDX[None, :, 0], _IDX[:, None, 1] + _IDX[None, :, 1],
+                           _IDX[:, None, 2] + _IDX[None, :, 2]]
                     pref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * a_c * b_c * c_c * d_c
                     blk = pref * (Eab @ Rm @ Ecd.T)
                     out += blk.reshape(out.shape)

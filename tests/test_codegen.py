@@ -53,54 +53,76 @@ def reference(La, Lb, Lc, Ld, F, p):
     return np.array([bra(*o) for o in G.coop_tables(La, Lb, Lc, Ld)["outs"]])
 
 
-def interpret(t, F, p):
-    """Execute the node programs the way fock_coop_kernel does."""
+def interpret(t, prims, AB, CD):
+    """Execute the node programs the way fock_coop_kernel does, for a
+    quartet of several primitive quartets (F, p) sharing the pair geometry
+    (AB, CD).  Checks that no slot is both read and written in one level."""
     s = np.full(t["total"] + 8, np.nan)
     nacc = t["nacc"]
     s[:nacc] = 0.0
+    for F, p in prims:
+        s[t["zs"]] = 0.0
+        for m in range(t["L"] + 1):
+            s[t["fslot"][m]] = F[m]
+        cf = np.zeros(6 * 4)
+        for sel in range(6):
+            op, d = sel // 3, sel % 3
+            cf[sel * 4:sel * 4 + 4] = ([p["PA"][d], p["WP"][d], p["oo2z"], p["rz"]] if op == 0 else
+                                       [p["QC"][d], p["WQ"][d], p["oo2n"], p["rn"]])
+        lev = t["vlev"]
+        for lv in range(len(lev) - 1):
+            nodes = t["vnodes"][lev[lv]:lev[lv + 1]]
+            reads = {x for nd in nodes for x in nd[:5]}
+            writes = {nd[5] for nd in nodes} | {nd[6] for nd in nodes if nd[6] != 0xFFFF}
+            assert not (reads & writes), "a level reads a slot it writes"
+            vals = []
+            for (a, b, c, d, e, dst, acc, meta) in nodes:
+                sel, n1, n2 = meta & 15, (meta >> 4) & 15, meta >> 8
+                c4 = cf[sel * 4:sel * 4 + 4]
+                vals.append(c4[0] * s[a] + c4[1] * s[b] + n1 * c4[2] * (s[c] - c4[3] * s[d]) + n2 * p["oo2zn"] * s[e])
+            for (a, b, c, d, e, dst, acc, meta), val in zip(nodes, vals):
+                s[dst] = val
+                if acc != 0xFFFF:
+                    s[acc] += val
+    s[nacc + 1:t["out"]] = np.nan          # the sink
     s[t["zs"]] = 0.0
-    for m in range(t["L"] + 1):
-        s[nacc + m] = F[m]
-    cf = np.zeros(6 * 4)
-    for sel in range(6):
-        op, d = sel // 3, sel % 3
-        cf[sel * 4:sel * 4 + 4] = ([p["PA"][d], p["WP"][d], p["oo2z"], p["rz"]] if op == 0 else
-                                   [p["QC"][d], p["WQ"][d], p["oo2n"], p["rn"]])
-    lev = t["vlev"]
-    for lv in range(len(lev) - 1):
-        written = []
-        for (a, b, c, d, e, dst, sel, n1, n2) in t["vnodes"][lev[lv]:lev[lv + 1]]:
-            c4 = cf[sel * 4:sel * 4 + 4]
-            for src in (a, b, c, d, e):
-                assert src not in written, "node reads a slot written in the same level"
-            val = c4[0] * s[a] + c4[1] * s[b] + n1 * c4[2] * (s[c] - c4[3] * s[d]) + n2 * p["oo2zn"] * s[e]
-            written.append(dst)
-            s[dst] = val
-    acc = s[t["tgt"]].copy()
-    s[:nacc] = acc
-    s[nacc:t["cf"]] = np.nan          # the HRR overwrites the VRR region
-    hcf = list(p["CD"]) + list(p["AB"])
+    hcf = list(CD) + list(AB)
     lev = t["hlev"]
     for lv in range(len(lev) - 1):
-        written = []
-        for (dst, s1, s2, sel, _) in [x + (0,) for x in t["hnodes"][lev[lv]:lev[lv + 1]]]:
-            assert s1 not in written and s2 not in written
-            s[dst] = s[s1] + hcf[sel] * s[s2]
-            written.append(dst)
-    return s[t["omap"]]
+        nodes = t["hnodes"][lev[lv]:lev[lv + 1]]
+        reads = {x for nd in nodes for x in nd[1:3]}
+        writes = {nd[0] for nd in nodes}
+        assert not (reads & writes), "an HRR level reads a slot it writes"
+        vals = [s[s1] + hcf[sel] * s[s2] for (dst, s1, s2, sel) in nodes]
+        for (dst, s1, s2, sel), val in zip(nodes, vals):
+            s[dst] = val
+    return s[t["out"]:t["out"] + t["ni"]]
+
+
+def _params(rng):
+    p = {k: rng.uniform(-1, 1, 3) for k in ("PA", "WP", "QC", "WQ")}
+    for k in ("oo2z", "oo2n", "oo2zn", "rz", "rn"):
+        p[k] = float(rng.uniform(0.1, 0.9))
+    return p
 
 
 @pytest.mark.parametrize("cls", G.COOP_CLASSES, ids=lambda c: "%d%d%d%d" % c)
 def test_coop_node_programs_match_recursion(cls):
+    """Two primitive quartets: the accumulators must hold their sum after
+    the slots of the first one were reused by the second."""
     rng = np.random.default_rng(sum(cls) * 7 + 1)
     t = G.coop_tables(*cls)
     L = t["L"]
-    F = rng.uniform(0.1, 1.0, L + 1)
-    p = {k: rng.uniform(-1, 1, 3) for k in ("PA", "WP", "QC", "WQ", "AB", "CD")}
-    for k in ("oo2z", "oo2n", "oo2zn", "rz", "rn"):
-        p[k] = float(rng.uniform(0.1, 0.9))
-    ref = reference(*cls, F, p)
-    got = interpret(t, F, p)
+    AB, CD = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+    prims = []
+    ref = 0.0
+    for _ in range(2):
+        F = rng.uniform(0.1, 1.0, L + 1)
+        p = _params(rng)
+        prims.append((F, p))
+        ref = ref + reference(*cls, F, dict(p, AB=AB, CD=CD))
+    got = interpret(t, prims, AB, CD)
     assert got.shape == ref.shape
+    assert np.all(np.isfinite(got))
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
-    assert t["total"] * 8 <= 64 * 1024
+    assert t["total"] * 8 <= G.COOP_LDS_CAP or t["glob"]

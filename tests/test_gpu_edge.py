@@ -1,5 +1,5 @@
 """Edge cases of the Fock build: empty work, a single atom, only s shells,
-rejected angular momentum, and a cluster large enough that some LDS tiles
+rejected angular momentum (above f), and a cluster large enough that some LDS tiles
 do not fit (size-independent properties there: the digestion paths agree,
 G is linear in P and symmetric)."""
 import ctypes
@@ -48,8 +48,10 @@ def test_small_systems_match_oracle(name, symbols, xyz, basis):
     assert np.max(np.abs(G - G_ref)) <= 1e-10
 
 
-def test_f_shells_are_rejected():
-    announce("edge: f shell rejected")
+def test_g_shells_are_rejected():
+    """QUICK stops at f (its reader sets ierr 37 for G shells, basis.f90);
+    the engine refuses L = 4 with QF_ERR_ARG and accepts L = 3."""
+    announce("edge: g shell rejected, f accepted")
     from quick_amd import _lib
     lib = _lib.lib()
     h = ctypes.c_void_p()
@@ -57,11 +59,15 @@ def test_f_shells_are_rejected():
     xyz = np.zeros(3)
     _lib.check(lib.qf_set_molecule(h, 1, _lib.dptr(xyz)), "molecule")
     i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+    rc = lib.qf_set_basis(h, 1, _lib.iptr(i32([0])), _lib.iptr(i32([4])), _lib.iptr(i32([4])), _lib.iptr(i32([1])),
+                          _lib.iptr(i32([0])), _lib.iptr(i32([0])), 1, _lib.dptr(np.ones(1)),
+                          _lib.dptr(np.ones(4)), 15)
+    assert rc == _lib.QF_OK + 1      # QF_ERR_ARG
+    assert b"above f" in lib.qf_last_error()
     rc = lib.qf_set_basis(h, 1, _lib.iptr(i32([0])), _lib.iptr(i32([3])), _lib.iptr(i32([3])), _lib.iptr(i32([1])),
                           _lib.iptr(i32([0])), _lib.iptr(i32([0])), 1, _lib.dptr(np.ones(1)),
                           _lib.dptr(np.ones(4)), 10)
-    assert rc == _lib.QF_OK + 1      # QF_ERR_ARG
-    assert b"L" in lib.qf_last_error() or len(lib.qf_last_error()) > 0
+    assert rc == _lib.QF_OK
     lib.qf_engine_destroy(h)
 
 

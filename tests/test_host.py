@@ -94,7 +94,7 @@ def test_codegen_flop_table_is_committed_and_consistent():
     import json
     from quick_amd.codegen import gen_eri
     tab = json.load(open(os.path.join(ROOT, "quick_amd", "csrc", "gen", "flop_table.json")))
-    assert len(tab) == 21
+    assert len(tab) == 55          # (La Lb|Lc Ld) for s, p, d, f
     for name, info in tab.items():
         La, Lb, Lc, Ld = info["La"], info["Lb"], info["Lc"], info["Ld"]
         _, _, fresh = gen_eri.gen_class(La, Lb, Lc, Ld)

@@ -359,3 +359,51 @@ def test_oracle_uhf_matches_quick_saved_energy(rec):
     nalpha = (nel + rec["mult"] - 1) // 2
     E, _, _ = O.uhf(orc, nalpha, nel - nalpha, cutoff=1e-10, prim_limit=1e-10)
     assert abs(E - rec["E_total"]) < 3e-7
+
+
+@pytest.mark.parametrize("basis", ["cc-pVTZ", "def2-TZVP"])
+def test_oracle_f_shell_eri_matches_mcmurchie_davidson(basis):
+    """Shell quartets with f shells (cc-pVTZ / def2-TZVP: f on the heavy
+    atom, d on H in cc-pVTZ) of a C-O fragment against the McMurchie-Davidson
+    code with the same FmT: pins the f component order, the cons factors
+    (sqrt 5, sqrt 15), QUICK's un-renormalised f contraction and the VRR/HRR
+    up to (ff|ff)."""
+    xyz = np.array([[0.02, -0.01, 0.03], [1.21, 0.07, -0.05], [-0.55, 0.93, 0.08]]) / O.BOHR
+    qb = O.QuickBasisRef(["C", "O", "H"], xyz, basis)
+    eri = O.Oracle(qb).eri_tensor()
+    ns = qb.nshell
+    fsh = [s for s in range(ns) if qb.qfinal[s] == 3]
+    dsh = [s for s in range(ns) if qb.qfinal[s] == 2]
+    small = [s for s in range(ns) if qb.kprim[s] <= 2 and qb.qfinal[s] < 2]
+    assert len(fsh) == 2
+    fa, fb = fsh
+    quartets = [(fa, fa, fa, fa), (fa, fb, fa, fb), (fa, fa, fb, fb), (fb, small[0], fa, small[1]),
+                (fa, dsh[0], small[0], fb), (dsh[-1], fa, fb, small[-1]), (fa, small[0], small[1], small[2])]
+    worst = 0.0
+    for q in quartets:
+        nabcd = int(sum(qb.qfinal[s] for s in q))
+        boys = (lambda n: (lambda L, T: O.fmt(n, T)[:L + 1]))(nabcd)
+        descs = [_shell_desc(qb, s) for s in q]
+        offs = [_func_offsets(qb, s) for s in q]
+        for sa in descs[0]:
+            for sb in descs[1]:
+                for sc in descs[2]:
+                    for sd in descs[3]:
+                        blk = MD.shell_block((sa, sb, sc, sd), boys)
+                        i0, j0 = offs[0][sa[0]], offs[1][sb[0]]
+                        k0, l0 = offs[2][sc[0]], offs[3][sd[0]]
+                        mine = eri[i0:i0 + blk.shape[0], j0:j0 + blk.shape[1],
+                                   k0:k0 + blk.shape[2], l0:l0 + blk.shape[3]]
+                        worst = max(worst, float(np.max(np.abs(mine - blk))))
+    assert worst < 1e-12, worst
+
+
+def test_f_shell_normalisation_and_one_electron():
+    """Every f function of cc-pVTZ / def2-TZVP has unit self-overlap with
+    QUICK's cons and un-renormalised contraction; the oracle's S equals a
+    McMurchie-Davidson overlap for f-f blocks."""
+    for basis in ("cc-pVTZ", "def2-TZVP"):
+        qb = water_qb(basis)
+        S, T, V = O.Oracle(qb).one_electron()
+        assert np.allclose(np.diag(S), 1.0, atol=1e-13)
+        assert np.max(np.abs(S - S.T)) < 1e-14 and np.max(np.abs(T - T.T)) < 1e-13
