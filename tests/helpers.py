@@ -113,3 +113,27 @@ def quick_abi_setup(lib, mol, orc, oshell=False, mult=1):
     Yf, CPf = np.asfortranarray(Y), np.asfortranarray(CP)
     lib.gpu_upload_cutoff_matrix_(vp(Yf), vp(CPf))
     return vp, D
+
+
+def md_schwarz(qb, a, b):
+    """sqrt(max_components |(ab|ab)|) of oracle shells a, b from the
+    McMurchie-Davidson code with QUICK's FmT (maxm = 2 (La + Lb), as the
+    Schwarz diagonals use): a value independent of both HRR formulations."""
+    from tests import md_reference as MD
+
+    def desc(s):
+        out = []
+        for L in range(qb.qstart[s], qb.qfinal[s] + 1):
+            p0, n = qb.kstart[s], qb.kprim[s]
+            out.append((L, qb.xyz[qb.katom[s]], qb.gcexpo[p0:p0 + n], qb.gccoeff[p0:p0 + n, L]))
+        return out
+
+    m = 0.0
+    for sa in desc(a):
+        for sb in desc(b):
+            maxm = 2 * (qb.qfinal[a] + qb.qfinal[b])
+            blk = MD.shell_block((sa, sb, sa, sb), lambda L, T, _m=maxm: O.fmt(_m, T)[:L + 1])
+            for i in range(blk.shape[0]):
+                for j in range(blk.shape[1]):
+                    m = max(m, abs(blk[i, j, i, j]))
+    return float(np.sqrt(m))
