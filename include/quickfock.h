@@ -98,6 +98,21 @@ int qf_get_stats(qf_engine *e, qf_stats *out);
 int qf_fock_build_device(qf_engine *e, const double *d_P, double *d_F, const double *d_cutmat,
                          void *stream);
 
+/* No counterpart in QUICK.  For callers that cannot allocate device memory
+ * during a build (HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same build as qf_fock_build_device (same kernels,
+ * same result), with its scratch (G partials, symmetrised G, the Cutmatrix
+ * when d_cutmat is NULL) in a fixed device array of QF_NOALLOC_SCRATCH_BYTES
+ * per GPU instead of engine buffers; no hipMalloc/hipFree, no HIP events.
+ * Limits: 2 nbasis^2 (+ nshell^2 without d_cutmat) doubles must fit, and no
+ * class may keep node-program slots in global memory ((ff|ff)-sized f
+ * classes); QF_ERR_ARG otherwise.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other engines) race, and G
+ * is only valid until the next no-alloc call (read it through d_F). */
+#define QF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int qf_fock_build_device_noalloc(qf_engine *e, const double *d_P, double *d_F, const double *d_cutmat,
+                                 void *stream);
+
 /* Host convenience: F[n*n] += G(P[n*n]); synchronous. */
 int qf_fock_build_host(qf_engine *e, const double *P, double *F, const double *cutmat);
 

@@ -59,6 +59,7 @@ def lib():
         "qf_prepare": (ctypes.c_int, [vp]),
         "qf_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(QfStats)]),
         "qf_fock_build_device": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "qf_fock_build_device_noalloc": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "qf_fock_build_host": (ctypes.c_int, [vp, dp, dp, dp]),
         "qf_fock_build_os_device": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
         "qf_fock_build_os_host": (ctypes.c_int, [vp, dp, dp, dp, dp, dp]),

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -268,7 +269,7 @@ struct qf_engine {
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
-    qf::MolDesc mol[2]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2
+    qf::MolDesc mol[3]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2, slot 2: no-alloc build
     // per-launch timing (HIP events on the build stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -884,6 +885,43 @@ int prepare(qf_engine* e)
     return QF_OK;
 }
 
+// Fixed scratch of qf_fock_build_device_noalloc: G partials, symmetrised G
+// and the Cutmatrix of one build live here instead of in buffers the
+// library allocates.  Each device holds its own copy for the life of the
+// process.  Host code never passes `scratch` itself to a kernel (on the host
+// it names a stand-in, not device memory): it uses the address
+// hipGetSymbolAddress returns, looked up once per device in
+// qf_engine_create with that device current.
+constexpr size_t kNoAllocDoubles = QF_NOALLOC_SCRATCH_BYTES / sizeof(double);
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(QF_ERR_BADDEVICE, "device id beyond the no-alloc table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return QF_OK;
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) return fail(QF_ERR_NODEVICE, "hipGetDevice");
+    if (hipSetDevice(device) != hipSuccess) return fail(QF_ERR_BADDEVICE, "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (err != hipSuccess || !p) return fail(QF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return QF_OK;
+}
+
+double* noalloc_scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 // One pass over every class launch: G(P) with the density screen dcm,
 // Coulomb terms on or off, then F += G and F2 += sign2 G.  `slot` picks the
 // device MolDesc, so two passes can be queued back to back on one stream.
@@ -1027,6 +1065,8 @@ int qf_engine_create(int device, qf_engine** out)
         if (hipGetDevice(&device) != hipSuccess) return fail(QF_ERR_NODEVICE, "hipGetDevice");
     }
     if (device >= count) return fail(QF_ERR_BADDEVICE, "device id out of range");
+    const int rc = noalloc_scratch_init(device);
+    if (rc) return rc;
     qf_engine* e = new qf_engine();
     e->device = device;
     *out = e;
@@ -1143,6 +1183,36 @@ int qf_fock_build_device(qf_engine* e, const double* d_P, double* d_F, const dou
     int rc = e->set_device();
     if (rc) return rc;
     return build(e, d_P, d_F, d_cutmat, (hipStream_t)stream);
+}
+
+int qf_fock_build_device_noalloc(qf_engine* e, const double* d_P, double* d_F, const double* d_cutmat, void* stream)
+{
+    if (!e || !d_P) return fail(QF_ERR_ARG, "NULL");
+    int rc = e->set_device();
+    if (rc) return rc;
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    double* base = noalloc_scratch_ptr(e->device);
+    if (!base) return fail(QF_ERR_STATE, "no-alloc scratch not set up on this device");
+    const size_t n2 = (size_t)e->nbasis * e->nbasis, ns2 = (size_t)e->nshell * e->nshell;
+    if (2 * n2 + (d_cutmat ? 0 : ns2) > (size_t)kNoAllocDoubles)
+        return fail(QF_ERR_ARG, "no-alloc build: nbasis too large for the fixed scratch (2 nbasis^2 + nshell^2 "
+                                "doubles must fit in QF_NOALLOC_SCRATCH_BYTES)");
+    for (const auto& L : e->launches)
+        if (L.tile == 2 && qf::coop_global_doubles(L.cb, L.ck) > 0)
+            return fail(QF_ERR_ARG, "no-alloc build: a class keeps its node-program slots in global memory");
+    const hipStream_t s = (hipStream_t)stream;
+    const int n = e->nbasis, ns = e->nshell;
+    // scratch layout: [G partials n^2 | symmetrised G n^2 | Cutmatrix nshell^2]
+    double* Gt = base;
+    double* Gs = base + n2;
+    if (!d_cutmat) {
+        double* cm = base + 2 * n2;
+        const int nthr = 256, tot = ns * ns;
+        hipLaunchKernelGGL(k_cutmat, dim3((tot + nthr - 1) / nthr), dim3(nthr), 0, s, d_P, n, ns, e->d_f0.p,
+                           e->d_f1.p, cm);
+        d_cutmat = cm;
+    }
+    return build_pass(e, d_P, d_cutmat, 1, d_F, nullptr, 0.0, 2, s, GBufs{Gt, 1, Gs, false});
 }
 
 int qf_fock_build_host(qf_engine* e, const double* P, double* F, const double* cutmat)

@@ -113,6 +113,16 @@ class FockEngine:
                                               ctypes.c_void_p(cutmat_ptr or 0), ctypes.c_void_p(stream or 0)),
                 "qf_fock_build_device")
 
+    def fock2e_device_noalloc(self, P_ptr, F_ptr=None, stream=None, cutmat_ptr=None):
+        """As fock2e_device, with the build's scratch in the library's fixed
+        device array (qf_fock_build_device_noalloc): no device allocation in
+        the call once the engine is prepared."""
+        self.prepare()
+        L.check(self.lib.qf_fock_build_device_noalloc(self.h, ctypes.c_void_p(P_ptr), ctypes.c_void_p(F_ptr or 0),
+                                                      ctypes.c_void_p(cutmat_ptr or 0),
+                                                      ctypes.c_void_p(stream or 0)),
+                "qf_fock_build_device_noalloc")
+
     def fock2e_os(self, Pa, Pb, cutmatrix=None):
         """Open-shell G for host densities (synchronous): (Ga, Gb) with
         Ga = J(Pa+Pb) - hyb K(Pa), Gb = J(Pa+Pb) - hyb K(Pb)."""
