@@ -113,6 +113,26 @@ int qf_fock_build_device(qf_engine *e, const double *d_P, double *d_F, const dou
 int qf_fock_build_device_noalloc(qf_engine *e, const double *d_P, double *d_F, const double *d_cutmat,
                                  void *stream);
 
+/* Two-electron nuclear gradient (QUICK's gpu_upload_grad_ + gpu_get_cshell_eri_grad_,
+ * gpu.cu:2136 and gpu_get2e_getxc_drivers.h:127; kernels getGrad_kernel_sp/_spd,
+ * gpu_get2e_subs_grad.h:1211-1650; CPU cshell_eri_grad, eri_grad.fh:43-768):
+ * grad[3*atom + xyz] += d/dR of E2 = 1/2 sum P G(P) with P held fixed, over the
+ * quartets with Ycut Ycut > grad_cutoff and Ycut Ycut DNmax > grad_cutoff (quartets on
+ * one atom skipped), primitive screen primLimit.  Shells up to d (QF_ERR_ARG with f).
+ * qf_set_grad_cutoff: QUICK's gradCutoff (default: the integral cutoff); a value below
+ * the integral cutoff makes the next qf_prepare extend the pair lists. */
+int qf_set_grad_cutoff(qf_engine *e, double grad_cutoff);
+/* lane = 1 (default): per-lane gradient kernels for the classes with La+Lb+Lc+Ld <= 3,
+ * the wave-per-quartet kernel for the rest; lane = 0: the wave-per-quartet kernel for all */
+int qf_set_grad_kernel(qf_engine *e, int lane);
+/* device pointers, asynchronous on stream; d_cutmat may be NULL (computed) */
+int qf_eri_grad_device(qf_engine *e, const double *d_P, double *d_grad, const double *d_cutmat, void *stream);
+/* host arrays, synchronous */
+int qf_eri_grad_host(qf_engine *e, const double *P, double *grad, const double *cutmat);
+/* per Fock launch: algorithmic FLOPs, quartets and primitive quartets of the last
+ * gradient (which must have computed its own Cutmatrix); any pointer may be NULL */
+int qf_grad_launch_flops(qf_engine *e, double *flops, long long *quartets, long long *prim_quartets);
+
 /* Host convenience: F[n*n] += G(P[n*n]); synchronous. */
 int qf_fock_build_host(qf_engine *e, const double *P, double *F, const double *cutmat);
 
@@ -286,6 +306,11 @@ void gpu_upload_cutoff_(double *cutMatrix, double *integralCutoff, double *primL
                         double *coreIntegralCutoff);
 /* gpu_get2e_getxc_drivers.h:27 gpu_get_cshell_eri_: o += G(dense), both triangles */
 void gpu_get_cshell_eri_(bool *deltaO, double *o);
+/* gpu.cu:2136 gpu_upload_grad_: QUICK's gradCutoff for the next gradient */
+void gpu_upload_grad_(double *gradCutoff);
+/* gpu_get2e_getxc_drivers.h:127 gpu_get_cshell_eri_grad_: grad(3, natom) += d/dR of the
+ * two-electron energy of the density of gpu_upload_density_matrix_ (s, p, d shells) */
+void gpu_get_cshell_eri_grad_(double *grad);
 /* gpu.cu:1771 gpu_upload_beta_density_matrix_ (the beta density of the next open-shell build) */
 void gpu_upload_beta_density_matrix_(double *denseb);
 /* gpu.cu:1728 gpu_upload_calculated_beta_ (only denseb is read) */

@@ -57,7 +57,11 @@ def lib():
         L.qo_get2e_os.restype = ctypes.c_long
         L.qo_get2e_os.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int, dp, dp]
+        L.qo_eri_grad.restype = ctypes.c_long
+        L.qo_eri_grad.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, dp]
         L.qo_fmt.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+        L.qo_set_fmt_series.argtypes = [ctypes.c_int]
         L.qo_one_electron.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.qo_eri_tensor.argtypes = [ctypes.c_void_p, dp]
         _lib = L
@@ -273,6 +277,20 @@ class Oracle:
         cmp = _p(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
         nq = lib().qo_get2e_os(self.h, _p(Pa), _p(Pb), cmp, cutoff, prim_limit, hyb, threads, _p(oa), _p(ob))
         return oa, ob, nq
+
+    def eri_grad(self, P, grad_cutoff=1e-12, prim_limit=None, hyb=1.0, threads=1, cutmatrix=None):
+        """d/dR of 1/2 sum P G(P) (P fixed) per atom, (natom, 3), and the
+        number of screened shell quartets (get_cshell_eri_grad)."""
+        if prim_limit is None:
+            prim_limit = 0.1 * grad_cutoff
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        g = np.zeros((len(self.qb.symbols), 3))
+        cm = None if cutmatrix is None else np.ascontiguousarray(cutmatrix, dtype=np.float64)
+        cmp = _p(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
+        nq = lib().qo_eri_grad(self.h, _p(P), cmp, grad_cutoff, prim_limit, hyb, threads, _p(g))
+        if nq < 0:
+            raise ValueError("the oracle's gradient covers shells up to d")
+        return g, nq
 
     def fock2e_shell_range(self, P, ii_begin, ii_end, cutoff=1e-12, prim_limit=None, hyb=1.0):
         if prim_limit is None:

@@ -93,6 +93,16 @@ struct qo_ctx {
 /* The fit coefficients are those of QUICK's routine; parity with QUICK      */
 /* requires them.                                                            */
 /* ------------------------------------------------------------------------- */
+/* Test switch: with it on, the small-X start value F_maxm comes from the
+ * convergent series sum_k (-X)^k / (k! (2 maxm + 2k + 1)) instead of QUICK's
+ * first-order (1 - X)/(2 maxm + 1).  Used only by the finite-difference
+ * checks of the gradient, where energy and gradient evaluate FmT with
+ * different maxm (NABCD and NABCD + 2) and QUICK's start value makes them
+ * disagree by ~1e-6. */
+static int g_fmt_series = 0;
+
+void qo_set_fmt_series(int on) { g_fmt_series = on; }
+
 void qo_fmt(int maxm, double X, double *vals)
 {
     const double PIE4 = QO_PI / 4.0;
@@ -146,7 +156,16 @@ void qo_fmt(int maxm, double X, double *vals)
                  1.99999999997023e-01) * X + 3.33333333333318e-01;
         WW1 = (X + X) * F1 + E;
     } else {
-        WW1 = (1.0 - X) / (double)(2 * maxm + 1);
+        if (g_fmt_series) {
+            double t = 1.0, acc = 0.0;
+            for (int k = 0; k < 30; ++k) {
+                acc += t / (double)(2 * maxm + 2 * k + 1);
+                t *= -X / (double)(k + 1);
+            }
+            WW1 = acc;
+        } else {
+            WW1 = (1.0 - X) / (double)(2 * maxm + 1);
+        }
     }
 
     if (X > 1.0e-1 || (X > 1.0e-04 && maxm < 4)) {
@@ -898,6 +917,253 @@ long qo_get2e_os(const qo_ctx *c, const double *Pa, const double *Pb, const doub
 /* ------------------------------------------------------------------------- */
 /* Brute-force integral tensor (tiny systems; digestion checks)              */
 /* ------------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------- */
+/* Two-electron gradient: get_cshell_eri_grad (gradient.fh:826-930),         */
+/* cshell_eri_grad (eri_grad.fh:43-199) and iclass_grad_cshell               */
+/* (eri_grad.fh:207-768).  Shells up to d: the raised VRR reaches            */
+/* [e|f] with |e|, |f| <= 5.                                                 */
+/* ------------------------------------------------------------------------- */
+#define GMAXL 2
+
+typedef struct {
+    int nprim;
+    int E, F;        /* Qfinal sums of bra and ket; the VRR runs to E+1, F+1 */
+    int *pidx;       /* nprim*4: Nii, Njj, Nkk, Nll */
+    double *Y;       /* nprim * NCUM(E+1) * NCUM(F+1) */
+} gprim_store;
+
+/* cshell_eri_grad's primitive loops (eri_grad.fh:97-170): FmT with
+ * NABCD = Qfinal sum + 2, VRR raised by one on both sides. */
+static void grad_shell_prims(const qo_ctx *c, int II, int JJ, int KK, int LL, double dnmax, double prim_limit,
+                             gprim_store *ps, double v[NCMAX][NCMAX][MMAX])
+{
+    const int jb = c->jbasis;
+    const double *RA = c->xyz + 3 * c->katom[II];
+    const double *RC = c->xyz + 3 * c->katom[KK];
+    const int E = c->qfinal[II] + c->qfinal[JJ], F = c->qfinal[KK] + c->qfinal[LL];
+    const int ne = NCUM(E + 1), nf = NCUM(F + 1);
+    double fm[MMAX + 5];
+    ps->E = E;
+    ps->F = F;
+    ps->nprim = 0;
+    for (int jjj = 0; jjj < c->kprim[JJ]; ++jjj) {
+        int Njj = c->kstart[JJ] + jjj;
+        for (int iii = 0; iii < c->kprim[II]; ++iii) {
+            int Nii = c->kstart[II] + iii;
+            double cutoffprim1 = dnmax * c->cutprim[Nii * jb + Njj];
+            for (int lll = 0; lll < c->kprim[LL]; ++lll) {
+                int Nll = c->kstart[LL] + lll;
+                for (int kkk = 0; kkk < c->kprim[KK]; ++kkk) {
+                    int Nkk = c->kstart[KK] + kkk;
+                    if (!(cutoffprim1 * c->cutprim[Nkk * jb + Nll] > prim_limit)) continue;
+                    vrr_par q;
+                    double T, prefac;
+                    prim_par(c, Nii, Njj, Nkk, Nll, RA, RC, &q, &T, &prefac);
+                    qo_fmt(E + F + 2, T, fm);
+                    vrr_full(E + 1, F + 1, fm, prefac, &q, v);
+                    int n = ps->nprim++;
+                    ps->pidx[n * 4 + 0] = Nii;
+                    ps->pidx[n * 4 + 1] = Njj;
+                    ps->pidx[n * 4 + 2] = Nkk;
+                    ps->pidx[n * 4 + 3] = Nll;
+                    double *dst = ps->Y + (size_t)n * ne * nf;
+                    for (int a = 0; a < ne; ++a)
+                        for (int f = 0; f < nf; ++f) dst[a * nf + f] = v[a][f][0];
+                }
+            }
+        }
+    }
+}
+
+/* (a b|c d) from a contracted store by the closed-form HRR on both sides,
+ * for explicit Cartesian exponents (hrrwholeopt). */
+static double hrr_lmn(const int *a, const int *b, const int *cc, const int *d, const double *AB, const double *CD,
+                      double store[NCMAX][NCMAX])
+{
+    double cl[64], cr[64];
+    int al[64], ar[64];
+    int nl = hrr_terms(a, b, AB, cl, al);
+    int nr = hrr_terms(cc, d, CD, cr, ar);
+    double Y = 0.0;
+    for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nr; ++j) Y += cl[i] * cr[j] * store[al[i]][ar[j]];
+    return Y;
+}
+
+typedef struct {
+    gprim_store ps;
+    double (*v)[NCMAX][MMAX];
+    double (*st)[NCMAX][NCMAX];   /* store, storeAA, storeBB, storeCC */
+} gwork_t;
+
+/* iclass_grad_cshell for one sub-class (I,J,K,L): the four contracted stores
+ * (weights 1, 2 AA, 2 BB, 2 CC; eri_grad.fh:238-309), then per Cartesian
+ * quartet of QUICK's visiting order (:357-766) the derivatives Yaa, Ybb, Ycc
+ * (hrrwholeopt: 2a (a+1|..) - n (a-1|..), times the cons factors) weighted
+ * with QUICK's density constant.  Every branch's constant equals
+ * (f/8) [4 P_ij P_kl - hyb (P_ik P_jl + P_il P_jk)], f the number of distinct
+ * index permutations of the quartet (8, or 4 / 2 for the coincidences QUICK
+ * enumerates). */
+static void iclass_grad(const qo_ctx *c, int II, int JJ, int KK, int LL, int I, int J, int K, int L,
+                        const gprim_store *ps, const double *P, double hyb, double *grad, gwork_t *w)
+{
+    const int n = c->nbasis, jb = c->jbasis;
+    const int ne = NCUM(ps->E + 1), nf = NCUM(ps->F + 1);
+    for (int t = 0; t < 4; ++t)
+        for (int a = 0; a < ne; ++a)
+            for (int f = 0; f < nf; ++f) w->st[t][a][f] = 0.0;
+    for (int k = 0; k < ps->nprim; ++k) {
+        const int *p = ps->pidx + k * 4;
+        double X44 = c->X0 * XC(c, p[0], p[1], I, J) * XC(c, p[2], p[3], K, L);
+        double wt[4] = {X44, X44 * c->gcexpo[p[0]] * 2.0, X44 * c->gcexpo[p[1]] * 2.0, X44 * c->gcexpo[p[2]] * 2.0};
+        const double *src = ps->Y + (size_t)k * ne * nf;
+        for (int t = 0; t < 4; ++t)
+            for (int a = 0; a < ne; ++a)
+                for (int f = 0; f < nf; ++f) w->st[t][a][f] += wt[t] * src[a * nf + f];
+    }
+    (void)jb;
+    const double *RA = c->xyz + 3 * c->katom[II], *RB = c->xyz + 3 * c->katom[JJ];
+    const double *RC = c->xyz + 3 * c->katom[KK], *RD = c->xyz + 3 * c->katom[LL];
+    double AB[3] = {RA[0] - RB[0], RA[1] - RB[1], RA[2] - RB[2]};
+    double CD[3] = {RC[0] - RD[0], RC[1] - RD[1], RC[2] - RD[2]};
+    int III1 = c->ksumtype[II] + c->qsbasis[II * 4 + I], III2 = c->ksumtype[II] + c->qfbasis[II * 4 + I];
+    int JJJ1 = c->ksumtype[JJ] + c->qsbasis[JJ * 4 + J], JJJ2 = c->ksumtype[JJ] + c->qfbasis[JJ * 4 + J];
+    int KKK1 = c->ksumtype[KK] + c->qsbasis[KK * 4 + K], KKK2 = c->ksumtype[KK] + c->qfbasis[KK * 4 + K];
+    int LLL1 = c->ksumtype[LL] + c->qsbasis[LL * 4 + L], LLL2 = c->ksumtype[LL] + c->qfbasis[LL * 4 + L];
+    const int general = II < JJ && II < KK && KK < LL;
+    double g[9] = {0};
+    for (int III = III1; III <= III2; ++III)
+        for (int JJJ = general ? JJJ1 : (III > JJJ1 ? III : JJJ1); JJJ <= JJJ2; ++JJJ)
+            for (int KKK = general ? KKK1 : (III > KKK1 ? III : KKK1); KKK <= KKK2; ++KKK)
+                for (int LLL = general ? LLL1 : (KKK > LLL1 ? KKK : LLL1); LLL <= LLL2; ++LLL) {
+                    if (!general && !(III < KKK) && !(JJJ <= LLL)) continue;
+                    if (!general && III == JJJ && III == KKK && III == LLL) continue;   /* (ii|ii): nothing */
+                    const double f = (III != JJJ ? 2.0 : 1.0) * (KKK != LLL ? 2.0 : 1.0) *
+                                     ((III != KKK || JJJ != LLL) ? 2.0 : 1.0);
+                    const double Pij = P[(size_t)JJJ * n + III], Pkl = P[(size_t)LLL * n + KKK];
+                    const double Pik = P[(size_t)KKK * n + III], Pjl = P[(size_t)LLL * n + JJJ];
+                    const double Pil = P[(size_t)LLL * n + III], Pjk = P[(size_t)KKK * n + JJJ];
+                    const double constant = f / 8.0 * (4.0 * Pij * Pkl - hyb * (Pik * Pjl + Pil * Pjk));
+                    const double cons = c->fn_cons[III] * c->fn_cons[JJJ] * c->fn_cons[KKK] * c->fn_cons[LLL];
+                    const int *a = c->fn_lmn + 3 * III, *b = c->fn_lmn + 3 * JJJ;
+                    const int *cc = c->fn_lmn + 3 * KKK, *d = c->fn_lmn + 3 * LLL;
+                    for (int x = 0; x < 3; ++x) {
+                        int ap[3] = {a[0], a[1], a[2]}, am_[3] = {a[0], a[1], a[2]};
+                        int bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
+                        int cp[3] = {cc[0], cc[1], cc[2]}, cm_[3] = {cc[0], cc[1], cc[2]};
+                        ap[x] += 1;
+                        bp[x] += 1;
+                        cp[x] += 1;
+                        double ya = hrr_lmn(ap, b, cc, d, AB, CD, w->st[1]);
+                        if (a[x] > 0) {
+                            am_[x] -= 1;
+                            ya -= a[x] * hrr_lmn(am_, b, cc, d, AB, CD, w->st[0]);
+                        }
+                        double yb = hrr_lmn(a, bp, cc, d, AB, CD, w->st[2]);
+                        if (b[x] > 0) {
+                            bm[x] -= 1;
+                            yb -= b[x] * hrr_lmn(a, bm, cc, d, AB, CD, w->st[0]);
+                        }
+                        double yc = hrr_lmn(a, b, cp, d, AB, CD, w->st[3]);
+                        if (cc[x] > 0) {
+                            cm_[x] -= 1;
+                            yc -= cc[x] * hrr_lmn(a, b, cm_, d, AB, CD, w->st[0]);
+                        }
+                        g[x] += constant * cons * ya;
+                        g[3 + x] += constant * cons * yb;
+                        g[6 + x] += constant * cons * yc;
+                    }
+                }
+    const int iA = c->katom[II], iB = c->katom[JJ], iC = c->katom[KK], iD = c->katom[LL];
+    for (int x = 0; x < 3; ++x) {
+        grad[iA * 3 + x] += g[x];
+        grad[iB * 3 + x] += g[3 + x];
+        grad[iC * 3 + x] += g[6 + x];
+        grad[iD * 3 + x] -= g[x] + g[3 + x] + g[6 + x];
+    }
+}
+
+/* get_cshell_eri_grad's loop for one II (gradient.fh:896-925). */
+static long grad_ii(const qo_ctx *c, int II, const double *cm, const double *P, double grad_cutoff,
+                    double prim_limit, double hyb, double *grad, gwork_t *w)
+{
+    const int ns = c->nshell;
+    long nq = 0;
+    for (int JJ = II; JJ < ns; ++JJ) {
+        double testtmp = c->Ycutoff[II * ns + JJ];
+        for (int KK = II; KK < ns; ++KK)
+            for (int LL = KK; LL < ns; ++LL) {
+                if (c->katom[II] == c->katom[JJ] && c->katom[II] == c->katom[KK] && c->katom[II] == c->katom[LL])
+                    continue;
+                double testCutoff = testtmp * c->Ycutoff[KK * ns + LL];
+                if (!(testCutoff > grad_cutoff)) continue;
+                double DNmax = 4.0 * cm[II * ns + JJ];
+                double t;
+                if ((t = 4.0 * cm[KK * ns + LL]) > DNmax) DNmax = t;
+                if ((t = cm[II * ns + LL]) > DNmax) DNmax = t;
+                if ((t = cm[II * ns + KK]) > DNmax) DNmax = t;
+                if ((t = cm[JJ * ns + KK]) > DNmax) DNmax = t;
+                if ((t = cm[JJ * ns + LL]) > DNmax) DNmax = t;
+                if (!(testCutoff * DNmax > grad_cutoff)) continue;
+                ++nq;
+                grad_shell_prims(c, II, JJ, KK, LL, DNmax, prim_limit, &w->ps, w->v);
+                for (int I = c->qstart[II]; I <= c->qfinal[II]; ++I)
+                    for (int J = c->qstart[JJ]; J <= c->qfinal[JJ]; ++J)
+                        for (int K = c->qstart[KK]; K <= c->qfinal[KK]; ++K)
+                            for (int L = c->qstart[LL]; L <= c->qfinal[LL]; ++L)
+                                iclass_grad(c, II, JJ, KK, LL, I, J, K, L, &w->ps, P, hyb, grad, w);
+            }
+    }
+    return nq;
+}
+
+long qo_eri_grad(const qo_ctx *c, const double *P, const double *cutmatrix, double grad_cutoff, double prim_limit,
+                 double hyb, int nthreads, double *grad)
+{
+    const int ns = c->nshell;
+    for (int s = 0; s < ns; ++s)
+        if (c->qfinal[s] > GMAXL) return -1;
+    double *cm = NULL;
+    if (!cutmatrix) {
+        cm = malloc(sizeof(double) * ns * ns);
+        qo_density_cutoff(c, P, cm);
+        cutmatrix = cm;
+    }
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    const size_t ng = (size_t)3 * c->natom;
+    double *part = calloc((size_t)nthreads * ng, sizeof(double));
+    long nq = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : nq)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        size_t mc4 = (size_t)c->maxcontract * c->maxcontract * c->maxcontract * c->maxcontract;
+        gwork_t w;
+        w.ps.pidx = malloc(sizeof(int) * 4 * mc4);
+        w.ps.Y = malloc(sizeof(double) * mc4 * NCUM(2 * GMAXL + 1) * NCUM(2 * GMAXL + 1));
+        w.v = malloc(sizeof(double) * NCMAX * NCMAX * MMAX);
+        w.st = malloc(sizeof(double) * 4 * NCMAX * NCMAX);
+#pragma omp for schedule(dynamic, 1)
+        for (int II = 0; II < ns; ++II)
+            nq += grad_ii(c, II, cutmatrix, P, grad_cutoff, prim_limit, hyb, part + (size_t)tid * ng, &w);
+        free(w.ps.pidx);
+        free(w.ps.Y);
+        free(w.v);
+        free(w.st);
+    }
+    for (int t = 0; t < nthreads; ++t)
+        for (size_t k = 0; k < ng; ++k) grad[k] += part[(size_t)t * ng + k];
+    free(part);
+    free(cm);
+    return nq;
+}
+
 void qo_eri_tensor(const qo_ctx *c, double *eri)
 {
     const int ns = c->nshell;

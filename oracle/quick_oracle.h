@@ -76,6 +76,18 @@ void qo_density_cutoff_os(const qo_ctx *c, const double *Pa, const double *Pb, d
 long qo_get2e_os(const qo_ctx *c, const double *Pa, const double *Pb, const double *cutmatrix, double cutoff,
                  double prim_limit, double hyb, int nthreads, double *oa, double *ob);
 
+/* Two-electron gradient (get_cshell_eri_grad / cshell_eri_grad / iclass_grad_cshell,
+ * gradient.fh:826-930, eri_grad.fh:43-768): grad[3*atom + xyz] += d/dR of
+ * 1/2 sum P G(P) with P fixed, quartets screened with grad_cutoff (quartets on one
+ * atom skipped).  cutmatrix NULL: qo_density_cutoff.  Shells up to d (returns -1
+ * otherwise); nthreads as qo_get2e_omp.  Returns the screened shell-quartet count. */
+long qo_eri_grad(const qo_ctx *c, const double *P, const double *cutmatrix, double grad_cutoff, double prim_limit,
+                 double hyb, int nthreads, double *grad);
+
+/* Test switch (default off = QUICK's FmT): the small-x start value from the
+ * convergent series instead of QUICK's first-order form. */
+void qo_set_fmt_series(int on);
+
 /* fmt.f90:13-85: F_m(x) for m = 0..maxm into vals[0..maxm]. */
 void qo_fmt(int maxm, double x, double *vals);
 

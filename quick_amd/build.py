@@ -34,6 +34,7 @@ FLAGS = [
 
 def sources():
     srcs = sorted(glob.glob(os.path.join(CSRC, "gen", "eri_*.hip")))
+    srcs += sorted(glob.glob(os.path.join(CSRC, "gen", "egrad_*.hip")))
     srcs += [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "oei.hip"),
              os.path.join(CSRC, "quick_compat.cpp")]
     return srcs
@@ -46,6 +47,11 @@ def _deps(src):
     if base.startswith("eri_"):
         deps.append(os.path.join(CSRC, "gen", base[4:-4] + ".h"))
         deps.append(os.path.join(CSRC, "gen", "coop_" + base[4:-4] + ".h"))
+    if base.startswith("egrad_"):
+        deps += [os.path.join(CSRC, "qf_grad.h"), os.path.join(CSRC, "gen", "grad_maps.h"),
+                 os.path.join(CSRC, "gen", "grad_" + base[6:-4] + ".h")]
+    if base == "engine.hip":
+        deps.append(os.path.join(CSRC, "gen", "grad_launchers.h"))
     return deps
 
 

@@ -741,6 +741,8 @@ def main():
             lines.append("    if (c == %d) return launch_schwarz_%s;" % (pc[(La, Lb)], name))
     lines += ["    return nullptr;", "}", "", "}  // namespace qf", ""]
     _write(os.path.join(OUT, "launchers.h"), "\n".join(lines))
+    from . import gen_grad
+    table["gradient"] = gen_grad.main(OUT, lambda name, text: _write(os.path.join(OUT, name), text))
     _write(os.path.join(OUT, "flop_table.json"), json.dumps(table, indent=1))
     for k, v in table.items():
         print(k, v)

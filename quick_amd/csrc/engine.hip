@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/quickfock.h"
+#include "gen/grad_launchers.h"
 #include "gen/launchers.h"
 #include "qf_types.h"
 
@@ -164,9 +165,11 @@ __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* 
 
 // Counts, per quartet class, the quartets and primitive quartets that pass
 // the screens (same tests as the Fock kernel); used for the FLOP model.
+// grad = 1: the gradient's screen (md.cutoff is gradCutoff, quartets on one
+// atom skipped).
 __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf::PairBlock* __restrict__ blocks,
                         const qf::MolDesc* __restrict__ mols, unsigned long long* __restrict__ counts, int slot,
-                        int fp32)
+                        int fp32, int grad)
 {
     const int tix = blockIdx.x;
     if (tix >= ntasks) return;
@@ -194,6 +197,10 @@ __global__ void k_count(const qf::Task* __restrict__ tasks, int ntasks, const qf
         if (!(y > md.cutoff && y * dn > md.cutoff)) continue;
         if (t.same_block && j > i) continue;
         if (fp32 ? !(y < md.split) : !(t.all_fp64 || y >= md.split)) continue;
+        if (grad) {
+            const int a = md.shell_atom[bi.sa];
+            if (a == md.shell_atom[bi.sb] && a == md.shell_atom[ki.sa] && a == md.shell_atom[ki.sb]) continue;
+        }
         if (t.pb == 0 && t.qb == 0) ++nq;   // a quartet spread over several tasks counts once
         for (int ip = t.pb; ip < t.pe; ++ip) {
             const double c1 = dn * bb.prim[(qf::PF_CUT * bb.K + ip) * bb.n + i];
@@ -221,6 +228,10 @@ struct qf_engine {
     std::vector<double> ycut, cutprim;
     bool have_schwarz = false;
     double cutoff = 1e-12, primlim = 1e-13, hyb = 1.0;
+    double grad_cutoff = -1.0;   // QUICK's gradCutoff (qf_set_grad_cutoff); <= 0: the integral cutoff
+    bool grad_lane = true;       // per-lane gradient kernels where a class has one (qf_set_grad_kernel)
+    // pair lists and task windows cover every quartet either screen can keep
+    double list_cutoff() const { return grad_cutoff > 0.0 ? std::min(cutoff, grad_cutoff) : cutoff; }
     int prec = QF_PREC_FP64;
     double split = 1e-4;
     double split_thr = 0.0;  // Ycut_ij*Ycut_kl below which quartets run in FP32
@@ -269,7 +280,12 @@ struct qf_engine {
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
-    qf::MolDesc mol[3]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2, slot 2: no-alloc build
+    DevBuf<int> d_shell_atom;    // original shell -> atom (gradient)
+    DevBuf<double> d_gscratch;   // global slots of the gradient classes that need them (on first use)
+    DevBuf<double> d_grad;       // gradient of the host entry point (on first use)
+    bool grad_built = false;     // a gradient has run since the last prepare() (counts need it) ...
+    bool grad_cm_own = false;    // ... screened with the engine's own Cutmatrix
+    qf::MolDesc mol[4]{};   // slot 0: the build (or open-shell pass 1), slot 1: open-shell pass 2, slot 2: no-alloc build, slot 3: gradient
     // per-launch timing (HIP events on the build stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -351,6 +367,7 @@ void make_pair(const qf_engine* e, int qa, int qb, int ip, int jp, bool use_cut,
             const double kpri = std::exp(-al * be / z * AB2) / z;
             p.v[qf::PF_KC] = sx0 * kpri * e->coef[(size_t)pa * 4 + a.L] * e->coef[(size_t)pb * 4 + b.L];
             p.v[qf::PF_CUT] = use_cut ? cutprim_sym(e, pa, pb) : 1.0;
+            p.v[qf::PF_ALPHA] = al;
             prims.push_back(p);
         }
     std::stable_sort(prims.begin(), prims.end(),
@@ -560,7 +577,7 @@ int prepare(qf_engine* e)
     int next_id = 0;
     for_each_canonical_pair(e, [&](int a, int b) {
         const double y = e->ycut[(size_t)e->qs[a].shell * e->nshell + e->qs[b].shell];
-        if (!(y * ymax > e->cutoff)) return;
+        if (!(y * ymax > e->list_cutoff())) return;
         qf::PairInfo info;
         std::vector<double> row;
         int K;
@@ -715,7 +732,7 @@ int prepare(qf_engine* e)
                             for (int k = 0; k < M; ++k) {
                                 const int i = B.bperm[g.first + k];
                                 const double yi = B.info[i].ycut;
-                                int n = prefix_count(Kb.info, yi, e->cutoff, true);
+                                int n = prefix_count(Kb.info, yi, e->list_cutoff(), true);
                                 if (same) n = std::min(n, i + 1);
                                 if (!fp32) e->stats.n_quartets_bound += n;
                                 int n64 = n;
@@ -871,7 +888,8 @@ int prepare(qf_engine* e)
     }
     QF_HIP(e->d_f0.upload(f0));
     QF_HIP(e->d_f1.upload(f1));
-    QF_HIP(e->d_mol.resize(3));   // slots: build / open-shell pass 1, open-shell pass 2, no-alloc build
+    QF_HIP(e->d_mol.resize(4));   // slots: build / open-shell pass 1, open-shell pass 2, no-alloc build, gradient
+    QF_HIP(e->d_shell_atom.upload(e->s_atom));
     QF_HIP(e->d_counts.resize(2 * std::max<size_t>(1, e->launches.size())));
     e->stats.device_bytes = (long long)(e->d_info.bytes() + e->d_prim.bytes() + e->d_tasks.bytes() +
                                         e->d_Gt.bytes() * 3 + e->d_cm.bytes() + e->d_blocks.bytes());
@@ -880,6 +898,8 @@ int prepare(qf_engine* e)
     e->ev_launch.clear();
     e->built = false;
     e->count_cm_own = false;
+    e->grad_built = false;
+    e->grad_cm_own = false;
     ++e->generation;
     e->prepared = true;
     return QF_OK;
@@ -1215,6 +1235,149 @@ int qf_fock_build_device_noalloc(qf_engine* e, const double* d_P, double* d_F, c
     return build_pass(e, d_P, d_cutmat, 1, d_F, nullptr, 0.0, 2, s, GBufs{Gt, 1, Gs, false});
 }
 
+// Two-electron gradient (QUICK's gpu_get_cshell_eri_grad_ -> getGrad,
+// gpu_get2e.cu:695-725): one wave-per-quartet gradient launch per class
+// over the tasks of the Fock build's launches (their union covers every
+// quartet in both precision halves), screened with gradCutoff.
+namespace {
+constexpr size_t kGradLdsMax = 64 * 1024;   // dynamic LDS of one gradient workgroup
+
+int grad_pass(qf_engine* e, const double* dP, const double* dcm, double* dgrad, hipStream_t s)
+{
+    // every class needs a gradient kernel: check before launching anything
+    size_t gdoubles = 0;
+    for (const auto& L : e->launches) {
+        if (!qf::grad_launcher(L.cb, L.ck))
+            return fail(QF_ERR_ARG, "ERI gradients cover shells up to d; this basis has f shells");
+        gdoubles = std::max<size_t>(gdoubles, (size_t)qf::grad_global_doubles(L.cb, L.ck));
+    }
+    if (gdoubles && e->d_gscratch.n < gdoubles * qf::COOP_GLOBAL_SLICES)
+        QF_HIP(e->d_gscratch.resize(gdoubles * qf::COOP_GLOBAL_SLICES));
+    qf::MolDesc md{};
+    md.P = dP;
+    md.PJ = dP;
+    md.G = nullptr;
+    md.cutmat = dcm;
+    md.nbasis = e->nbasis;
+    md.nshell = e->nshell;
+    md.cutoff = e->grad_cutoff > 0.0 ? e->grad_cutoff : e->cutoff;
+    md.primlim = e->primlim;
+    md.hyb = e->hyb;
+    md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;   // only separates the launches' halves
+    md.replicas = 1;
+    md.natom = e->natom;
+    md.shell_atom = e->d_shell_atom.p;
+    md.grad = dgrad;
+    e->mol[3] = md;
+    qf::MolDesc* dmol = e->d_mol.p + 3;
+    QF_HIP(hipMemcpyAsync(dmol, &e->mol[3], sizeof(md), hipMemcpyHostToDevice, s));
+    const size_t tile_bytes = (size_t)3 * e->natom * sizeof(double);
+    for (const auto& L : e->launches) {
+        const qf::GradLaunch lane = e->grad_lane ? qf::grad_lane_launcher(L.cb, L.ck) : nullptr;
+        const size_t slots = lane ? 0 : (size_t)qf::grad_lds_doubles(L.cb, L.ck) * sizeof(double);
+        const size_t tile = slots + tile_bytes <= kGradLdsMax ? tile_bytes : 0;
+        // one workgroup per task unless the engine caps the grid (qf_set_persistent):
+        // the hardware then hands out tasks as workgroups finish
+        const int grid = e->persist > 0 ? std::min(L.count, e->persist) : L.count;
+        (lane ? lane : qf::grad_launcher(L.cb, L.ck))(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, grid,
+                                                      tile, e->d_gscratch.p, L.fp32, s);
+    }
+    QF_HIP(hipGetLastError());
+    return QF_OK;
+}
+}  // namespace
+
+int qf_set_grad_cutoff(qf_engine* e, double grad_cutoff)
+{
+    if (!e || !(grad_cutoff > 0.0)) return fail(QF_ERR_ARG, "gradient cutoff must be positive");
+    const double before = e->list_cutoff();
+    e->grad_cutoff = grad_cutoff;
+    if (e->list_cutoff() != before) e->prepared = false;   // the lists must reach further down
+    return QF_OK;
+}
+
+int qf_set_grad_kernel(qf_engine* e, int lane)
+{
+    if (!e) return fail(QF_ERR_ARG, "engine is NULL");
+    e->grad_lane = lane != 0;
+    return QF_OK;
+}
+
+int qf_eri_grad_device(qf_engine* e, const double* d_P, double* d_grad, const double* d_cutmat, void* stream)
+{
+    if (!e || !d_P || !d_grad) return fail(QF_ERR_ARG, "NULL");
+    int rc = e->set_device();
+    if (rc) return rc;
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    const hipStream_t s = (hipStream_t)stream;
+    const double* dcm = d_cutmat;
+    if (!dcm) {
+        const int nthr = 256, tot = e->nshell * e->nshell;
+        hipLaunchKernelGGL(k_cutmat, dim3((tot + nthr - 1) / nthr), dim3(nthr), 0, s, d_P, e->nbasis, e->nshell,
+                           e->d_f0.p, e->d_f1.p, e->d_cm.p);
+        dcm = e->d_cm.p;
+    }
+    rc = grad_pass(e, d_P, dcm, d_grad, s);
+    if (rc == QF_OK) {
+        e->grad_built = true;
+        e->grad_cm_own = (dcm == e->d_cm.p);
+    }
+    return rc;
+}
+
+int qf_eri_grad_host(qf_engine* e, const double* P, double* grad, const double* cutmat)
+{
+    if (!e || !P || !grad) return fail(QF_ERR_ARG, "NULL");
+    int rc = e->set_device();
+    if (rc) return rc;
+    if (!e->prepared) return fail(QF_ERR_STATE, "qf_prepare has not been called");
+    const size_t n2 = (size_t)e->nbasis * e->nbasis, ng = (size_t)3 * e->natom;
+    QF_HIP(hipMemcpy(e->d_P.p, P, n2 * sizeof(double), hipMemcpyHostToDevice));
+    const double* dcm = nullptr;
+    if (cutmat) {
+        QF_HIP(hipMemcpy(e->d_cm.p, cutmat, (size_t)e->nshell * e->nshell * sizeof(double), hipMemcpyHostToDevice));
+        dcm = e->d_cm.p;
+    }
+    if (e->d_grad.n < ng) QF_HIP(e->d_grad.resize(ng));
+    QF_HIP(hipMemset(e->d_grad.p, 0, ng * sizeof(double)));
+    rc = qf_eri_grad_device(e, e->d_P.p, e->d_grad.p, dcm, nullptr);
+    if (rc) return rc;
+    std::vector<double> g(ng);
+    QF_HIP(hipMemcpy(g.data(), e->d_grad.p, ng * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ng; ++k) grad[k] += g[k];
+    return QF_OK;
+}
+
+int qf_grad_launch_flops(qf_engine* e, double* flops, long long* quartets, long long* prims)
+{
+    // counts, with the density screen of the last gradient, the quartets and
+    // primitive quartets the gradient launches evaluate, priced with the
+    // generated per-class gradient FLOP table
+    if (!e || !e->prepared) return fail(QF_ERR_STATE, "not prepared");
+    if (!e->grad_built) return fail(QF_ERR_STATE, "no gradient since the last qf_prepare");
+    if (!e->grad_cm_own)
+        return fail(QF_ERR_STATE, "the last gradient screened with a caller-owned Cutmatrix; counts need one that "
+                                  "computes its own (d_cutmat = NULL)");
+    int rc = e->set_device();
+    if (rc) return rc;
+    const size_t nl = e->launches.size();
+    QF_HIP(hipMemset(e->d_counts.p, 0, 2 * std::max<size_t>(nl, 1) * sizeof(unsigned long long)));
+    for (size_t k = 0; k < nl; ++k) {
+        const auto& L = e->launches[k];
+        hipLaunchKernelGGL(k_count, dim3(L.count), dim3(qf::WAVE), 0, 0, e->d_tasks.p + L.offset, L.count,
+                           e->d_blocks.p, e->d_mol.p + 3, e->d_counts.p, (int)k, L.fp32, 1);
+    }
+    QF_HIP(hipGetLastError());
+    std::vector<unsigned long long> c(2 * std::max<size_t>(nl, 1));
+    QF_HIP(hipMemcpy(c.data(), e->d_counts.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nl; ++k) {
+        if (flops) flops[k] = qf::grad_class_flops(e->launches[k].cb, e->launches[k].ck, c[2 * k], c[2 * k + 1]);
+        if (quartets) quartets[k] = (long long)c[2 * k];
+        if (prims) prims[k] = (long long)c[2 * k + 1];
+    }
+    return QF_OK;
+}
+
 int qf_fock_build_host(qf_engine* e, const double* P, double* F, const double* cutmat)
 {
     if (!e || !P || !F) return fail(QF_ERR_ARG, "NULL");
@@ -1347,7 +1510,7 @@ int qf_launch_flops(qf_engine* e, double* flops)
     for (size_t k = 0; k < nl; ++k) {
         const auto& L = e->launches[k];
         hipLaunchKernelGGL(k_count, dim3(L.count), dim3(qf::WAVE), 0, 0, e->d_tasks.p + L.offset, L.count,
-                           e->d_blocks.p, e->d_mol.p, e->d_counts.p, (int)k, L.fp32);
+                           e->d_blocks.p, e->d_mol.p, e->d_counts.p, (int)k, L.fp32, 0);
     }
     QF_HIP(hipGetLastError());
     std::vector<unsigned long long> c(2 * std::max<size_t>(nl, 1));

@@ -117,7 +117,10 @@ __host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
     } while (0)
 #define QF_BOYS_LDS_PTR ((const QF_AS_LDS double*)qf_boys_tab)
 
-template <int L, typename Tab>
+// EXTRA: how far maxm may exceed L on the downward branch (L + EXTRA + 1 at
+// most): 3 for the Fock classes (the s parts of SP shells), 4 for the
+// gradient, whose maxm carries QUICK's extra +1 (eri_grad.fh:104-106).
+template <int L, typename Tab, int EXTRA = 3>
 __host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1], Tab tab)
 {
     const double XINV = qf_rcp(X);
@@ -143,7 +146,7 @@ __host__ __device__ __forceinline__ void boys_quick(const double X, const int ma
         const double twot = X * 2.0;
         double f = (1.0 - X) * qf_rcp(2 * maxm + 1.0);
 #pragma unroll
-        for (int m = L + 3; m >= L; --m)
+        for (int m = L + EXTRA; m >= L; --m)
             if (m < maxm) f = (twot * f + E) * (1.0 / (2 * m + 1));
         F[L] = f;
 #pragma unroll

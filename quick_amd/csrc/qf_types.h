@@ -16,7 +16,9 @@ constexpr int WAVE = 64;   // lanes per wavefront on CDNA
 // slots in global memory: the slot buffer holds this many slices.
 constexpr int COOP_GLOBAL_SLICES = 1024;
 
-enum PrimField { PF_ZETA = 0, PF_OOZ2, PF_PX, PF_PY, PF_PZ, PF_KC, PF_CUT, PF_N };
+// PF_ALPHA: exponent of the first Q-shell's primitive (the second's is
+// zeta - alpha); read only by the gradient kernels.
+enum PrimField { PF_ZETA = 0, PF_OOZ2, PF_PX, PF_PY, PF_PZ, PF_KC, PF_CUT, PF_ALPHA, PF_N };
 
 struct PairInfo {
     int qa, qb;      // Q-shell ids (an SP shell is split into an s and a p Q-shell)
@@ -49,7 +51,9 @@ struct MolDesc {
     double cutoff, primlim, hyb;
     double split;           // quartets with Ycut*Ycut >= split run in FP64, below in FP32
     int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
-    int pad;
+    int natom;              // gradient: atoms (3 natom entries of grad)
+    const int* shell_atom;  // gradient: original shell -> atom
+    double* grad;           // gradient: 3 natom accumulator, x y z per atom
 };
 
 // One wave: bras bperm[kb..ke) of one bra block - all with the same first

@@ -293,6 +293,35 @@ void gpu_get_oshell_eri_(bool* /*deltaO*/, double* o, double* ob)
         fatal("gpu_get_oshell_eri_");
 }
 
+void gpu_upload_grad_(double* gradCutoff)
+{
+    // QUICK allocates the device gradient here; the shim only records the
+    // cutoff (a lower one reaches further down the pair lists at the next prepare)
+    if (!g_state || !g_state->eng) return;
+    if (qf_set_grad_cutoff(g_state->eng, *gradCutoff)) report("gpu_upload_grad_");
+}
+
+void gpu_get_cshell_eri_grad_(double* grad)
+{
+    // grad(3, natom) += d/dR of the two-electron energy of dense, with the
+    // Cutmatrix of the last gpu_upload_cutoff_ (gradient.fh:575-584, 867)
+    if (!g_state || !g_state->eng) {
+        std::fprintf(stderr, "quickfock: gpu_get_cshell_eri_grad_ called before gpu_new_/gpu_upload_basis_\n");
+        std::exit(EXIT_FAILURE);
+    }
+    if (g_state->oshell) {
+        std::fprintf(stderr, "quickfock: gpu_get_cshell_eri_grad_ called for an open-shell method\n");
+        std::exit(EXIT_FAILURE);
+    }
+    if (!g_state->cut_set) {
+        qf_set_cutoffs(g_state->eng, g_state->cutoff, g_state->primlim, g_state->hyb);
+        g_state->cut_set = true;
+    }
+    if (qf_prepare(g_state->eng) != QF_OK) fatal("gpu_get_cshell_eri_grad_ (prepare)");
+    const double* cm = g_state->cutmat.empty() ? nullptr : g_state->cutmat.data();
+    if (qf_eri_grad_host(g_state->eng, g_state->dense.data(), grad, cm)) fatal("gpu_get_cshell_eri_grad_");
+}
+
 void gpu_cleanup_(void) {}
 
 void gpu_delete_(int* ierr)

@@ -123,6 +123,44 @@ class FockEngine:
                                                       ctypes.c_void_p(stream or 0)),
                 "qf_fock_build_device_noalloc")
 
+    def set_grad_cutoff(self, grad_cutoff):
+        """QUICK's gradCutoff for the two-electron gradient (default: the
+        integral cutoff)."""
+        L.check(self.lib.qf_set_grad_cutoff(self.h, float(grad_cutoff)), "qf_set_grad_cutoff")
+
+    def set_grad_kernel(self, lane=True):
+        """Per-lane gradient kernels for the light classes (default), or the
+        wave-per-quartet kernel for every class."""
+        L.check(self.lib.qf_set_grad_kernel(self.h, 1 if lane else 0), "qf_set_grad_kernel")
+
+    def eri_grad(self, P, cutmatrix=None):
+        """d/dR of 1/2 sum P G(P) with P fixed, (natom, 3) in Eh/bohr (QUICK's
+        gpu_get_cshell_eri_grad_); synchronous."""
+        self.prepare()
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        g = np.zeros((self.basis.mol.natom, 3))
+        cm = None if cutmatrix is None else np.ascontiguousarray(cutmatrix, dtype=np.float64)
+        cmp = L.dptr(cm) if cm is not None else ctypes.POINTER(ctypes.c_double)()
+        L.check(self.lib.qf_eri_grad_host(self.h, L.dptr(P), L.dptr(g), cmp), "qf_eri_grad_host")
+        return g
+
+    def eri_grad_device(self, P_ptr, grad_ptr, stream=None, cutmat_ptr=None):
+        """grad (3 natom doubles on the device) += the two-electron gradient;
+        asynchronous on ``stream``."""
+        self.prepare()
+        L.check(self.lib.qf_eri_grad_device(self.h, ctypes.c_void_p(P_ptr), ctypes.c_void_p(grad_ptr),
+                                            ctypes.c_void_p(cutmat_ptr or 0), ctypes.c_void_p(stream or 0)),
+                "qf_eri_grad_device")
+
+    def grad_launch_counts(self):
+        """Per Fock launch of the last gradient: (FLOPs, quartets, primitive quartets)."""
+        n = self.lib.qf_launch_count(self.h)
+        f = np.zeros(max(n, 1))
+        q = (ctypes.c_longlong * max(n, 1))()
+        p = (ctypes.c_longlong * max(n, 1))()
+        L.check(self.lib.qf_grad_launch_flops(self.h, L.dptr(f), q, p), "qf_grad_launch_flops")
+        return f[:n], np.array(q[:n]), np.array(p[:n])
+
     def fock2e_os(self, Pa, Pb, cutmatrix=None):
         """Open-shell G for host densities (synchronous): (Ga, Gb) with
         Ga = J(Pa+Pb) - hyb K(Pa), Gb = J(Pa+Pb) - hyb K(Pb)."""

@@ -94,7 +94,13 @@ def test_codegen_flop_table_is_committed_and_consistent():
     import json
     from quick_amd.codegen import gen_eri
     tab = json.load(open(os.path.join(ROOT, "quick_amd", "csrc", "gen", "flop_table.json")))
+    grad = tab.pop("gradient")
     assert len(tab) == 55          # (La Lb|Lc Ld) for s, p, d, f
+    assert len(grad) == 21         # gradient classes: s, p, d
+    from quick_amd.codegen import gen_grad
+    for name, info in grad.items():
+        t = gen_grad.grad_tables(*(int(ch) for ch in name[1:]))
+        assert (t["prim_flop"], t["quartet_flop"]) == (info["prim_flop"], info["quartet_flop"])
     for name, info in tab.items():
         La, Lb, Lc, Ld = info["La"], info["Lb"], info["Lc"], info["Ld"]
         _, _, fresh = gen_eri.gen_class(La, Lb, Lc, Ld)
