@@ -78,7 +78,10 @@ def main():
     ap.add_argument("--molecules", type=int, default=256)
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
-    ap.add_argument("--autotune", type=int, default=1, help="tune launch shapes per class before timing")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="launch shapes per class: 1 = the committed table (quick_amd/tuning/autotune.json) "
+                         "when it has this device and workload, else tune; 2 = tune again; 0 = engine defaults")
+    ap.add_argument("--tune-out", default=None, help="write the settings tuned in this run to this JSON table")
     ap.add_argument("--prim-split", type=int, default=0,
                     help="primitive pairs per task piece (0: automatic, < 0: never split)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -205,11 +208,12 @@ def measure_h2o32(a, dev):
     sh = stream.cuda_stream
     tuned = None
     if a.autotune:
-        from quick_amd.autotune import autotune
+        from quick_amd.autotune import tune_or_load
         ta = time.perf_counter()
-        tuned = autotune(eng, sh, builds=2, make_density=lambda i: P.data_ptr())
+        tuned, how = tune_or_load(eng, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32", builds=2,
+                                  make_density=lambda i: P.data_ptr())
         torch.cuda.synchronize()
-        log(f"[bench] autotune {time.perf_counter() - ta:.2f}s: " +
+        log(f"[bench] autotune ({how}) {time.perf_counter() - ta:.2f}s: " +
             ", ".join(f"{k}:{v[0]}" for k, v in sorted(tuned.items())))
 
     def timed(fn, steps):
@@ -313,7 +317,7 @@ def measure_gly10(a, dev):
     import torch
     from quick_amd import basis as B
     from quick_amd import molecules as M
-    from quick_amd.autotune import autotune
+    from quick_amd.autotune import tune_or_load
     from quick_amd.engine import FockEngine
     from quick_amd.scf import rhf
     mol = M.polyglycine(10)
@@ -332,7 +336,8 @@ def measure_gly10(a, dev):
     rate, G = {}, {}
     for prec, e in engs.items():
         if a.autotune:
-            autotune(e, sh, builds=2, make_density=lambda i: Pd.data_ptr())
+            tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="gly10", builds=2,
+                         make_density=lambda i: Pd.data_ptr())
         t = _timed(lambda: e.fock2e_device(Pd.data_ptr(), Fd.data_ptr(), sh), a.steps, a.warmup)
         rate[prec] = 1.0 / t
         G[prec] = e.fock2e(P64)
@@ -364,7 +369,7 @@ def measure_batch(a, dev):
     import torch
     from quick_amd import basis as B
     from quick_amd import molecules as M
-    from quick_amd.autotune import autotune
+    from quick_amd.autotune import tune_or_load
     from quick_amd.engine import FockBatch, FockEngine
     from quick_amd.scf import seeded_density
     mols = M.mixed_set(a.molecules)
@@ -383,7 +388,8 @@ def measure_batch(a, dev):
     res = {}
     for name, engs in (("fp64", e64), ("mixed", emx)):
         if a.autotune:
-            autotune(engs, s, builds=1, batch_kw={"replicas": 2}, densities=dens)
+            tune_or_load(engs, s, batched=True, retune=a.autotune == 2, save_path=a.tune_out, tag="mixed256",
+                         builds=1, batch_kw={"replicas": 2}, densities=dens)
         b = FockBatch(engs, replicas=2, graph=True)
         for i, P in enumerate(dens):
             b.set_density(i, P)
@@ -514,7 +520,7 @@ def run_batch(a, dev):
     import torch
     from quick_amd import basis as B
     from quick_amd import molecules as M
-    from quick_amd.autotune import autotune
+    from quick_amd.autotune import tune_or_load
     from quick_amd.engine import FockBatch, FockEngine
     from quick_amd.scf import seeded_density
     mols = M.mixed_set(a.molecules)
@@ -542,7 +548,8 @@ def run_batch(a, dev):
     for name, engs in (("fp64", e64), ("mixed", emx)):
         tuned = None
         if a.autotune:
-            tuned = autotune(engs, s, builds=1, batch_kw={"replicas": 2}, densities=dens)
+            tuned, _ = tune_or_load(engs, s, batched=True, retune=a.autotune == 2, save_path=a.tune_out,
+                                    tag="mixed256", builds=1, batch_kw={"replicas": 2}, densities=dens)
         b = FockBatch(engs, replicas=2, graph=True)
         for i, P in enumerate(dens):
             b.set_density(i, P)
@@ -651,7 +658,7 @@ def run_uhf(a, dev):
     import torch
     from quick_amd import basis as B
     from quick_amd import molecules as M
-    from quick_amd.autotune import autotune
+    from quick_amd.autotune import tune_or_load
     from quick_amd.engine import FockEngine
     from quick_amd.scf import uhf
     m0 = M.water_cluster(a.nwater)
@@ -665,7 +672,8 @@ def run_uhf(a, dev):
     Pt = Pa + Pb
     sh = torch.cuda.current_stream(dev).cuda_stream
     if a.autotune:
-        autotune(e, sh, builds=2, make_density=lambda i: Pt.data_ptr())
+        tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32+", builds=2,
+                     make_density=lambda i: Pt.data_ptr())
     Fa, Fb, F = torch.zeros_like(Pa), torch.zeros_like(Pa), torch.zeros_like(Pa)
     t_os = _timed(lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh),
                   a.steps, a.warmup)

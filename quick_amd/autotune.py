@@ -9,6 +9,9 @@ on a list of engines that are built as a FockBatch.
 """
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 
 # (digestion, waves, tasks_per_group, bras_per_task); digestion 0 direct
@@ -115,3 +118,78 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
         chosen[name] = (tuple(cand) + (best,), ptotal(best) if pieces else chosen[name][1])
     autotune.last_piece_times = ptimes
     return chosen
+
+
+# ---------------------------------------------------------------------------
+# Persisted tables: the chosen settings per class, keyed by device, build
+# kind, precision and molecule size, so a run can reuse them instead of
+# tuning again.  quick_amd/tuning/autotune.json holds the committed table.
+# ---------------------------------------------------------------------------
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "autotune.json")
+
+
+def size_bucket(nbasis):
+    """The engine's own size classes (primitive piece defaults): <=128, <=256, more."""
+    return "n<=128" if nbasis <= 128 else "n<=256" if nbasis <= 256 else "n>256"
+
+
+def table_key(engines, batched=False, device_name=None, tag=None):
+    """device|single or batch|precision|size bucket[|tag]; tag names a
+    workload whose settings should not be shared with others of its size."""
+    engs = engines if isinstance(engines, (list, tuple)) else [engines]
+    if device_name is None:
+        try:
+            import torch
+            device_name = torch.cuda.get_device_name(engs[0].device if hasattr(engs[0], "device") else 0)
+        except Exception:
+            device_name = "unknown"
+    prec = getattr(engs[0], "precision", "fp64")
+    n = max(e.n for e in engs)
+    return "|".join([device_name, "batch" if batched else "single", prec, size_bucket(n)] + ([tag] if tag else []))
+
+
+def load_table(key, path=TABLE):
+    """{class name: (digestion, waves, tasks/group, bras/task, piece)} or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tab = json.load(fh)
+    entry = tab.get(key)
+    return {k: tuple(v) for k, v in entry.items()} if entry else None
+
+
+def save_table(key, chosen, path=TABLE):
+    """Stores an autotune() result under key (other keys are kept)."""
+    tab = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            tab = json.load(fh)
+    tab[key] = {name: [int(x) for x in cand] for name, (cand, _) in sorted(chosen.items())}
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as fh:
+        json.dump(tab, fh, indent=1, sort_keys=True)
+
+
+def apply_table(engines, entry):
+    engs = engines if isinstance(engines, (list, tuple)) else [engines]
+    for name, cand in entry.items():
+        cb, ck = _class_index(name)
+        for e in engs:
+            e.set_class_tuning(cb, ck, *cand[:4])
+            e.set_class_piece(cb, ck, cand[4])
+
+
+def tune_or_load(engines, stream, batched=False, retune=False, path=TABLE, save_path=None, tag=None, **kw):
+    """Applies the persisted settings for this key when the table has them
+    (returns (entry, "table")), otherwise runs autotune() and stores the
+    result in save_path (returns (chosen, "tuned"))."""
+    key = table_key(engines, batched, tag=tag)
+    if not retune:
+        entry = load_table(key, path)
+        if entry:
+            apply_table(engines, entry)
+            return {k: (v, None) for k, v in entry.items()}, "table"
+    chosen = autotune(engines, stream, **kw)
+    if save_path:
+        save_table(key, chosen, save_path)
+    return chosen, "tuned"

@@ -107,3 +107,18 @@ def test_codegen_flop_table_is_committed_and_consistent():
         assert fresh["vrr_flop"] == info["vrr_flop"] and fresh["hrr_flop"] == info["hrr_flop"]
         nc = lambda L: (L + 1) * (L + 2) // 2
         assert info["ni"] == nc(La) * nc(Lb) * nc(Lc) * nc(Ld)
+
+
+def test_autotune_table_round_trip(tmp_path):
+    """A tuned entry is stored under its key, other keys survive, and the
+    loaded entry is what was stored."""
+    from quick_amd import autotune as A
+    path = str(tmp_path / "tab.json")
+    chosen = {"(ss|ss)": ((1, 4, 8, 0, 16), 1.5), "(dd|dd)": ((2, 0, 0, 1, 8), 0.2)}
+    A.save_table("dev|single|fp64|n>256|x", chosen, path)
+    A.save_table("dev|batch|mixed|n<=128", {"(ps|ss)": ((0, 0, 0, 2, -1), 0.1)}, path)
+    got = A.load_table("dev|single|fp64|n>256|x", path)
+    assert got == {"(ss|ss)": (1, 4, 8, 0, 16), "(dd|dd)": (2, 0, 0, 1, 8)}
+    assert A.load_table("dev|batch|mixed|n<=128", path) == {"(ps|ss)": (0, 0, 0, 2, -1)}
+    assert A.load_table("missing", path) is None
+    assert A.size_bucket(128) == "n<=128" and A.size_bucket(200) == "n<=256" and A.size_bucket(608) == "n>256"
