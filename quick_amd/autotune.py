@@ -140,7 +140,8 @@ def table_key(engines, batched=False, device_name=None, tag=None):
     if device_name is None:
         try:
             import torch
-            device_name = torch.cuda.get_device_name(engs[0].device if hasattr(engs[0], "device") else 0)
+            props = torch.cuda.get_device_properties(0)
+            device_name = getattr(props, "gcnArchName", "").split(":")[0] or props.name
         except Exception:
             device_name = "unknown"
     prec = getattr(engs[0], "precision", "fp64")
