@@ -692,7 +692,13 @@ int prepare(qf_engine* e)
                 // classes without per-lane kernels (La+Lb+Lc+Ld >= 6) run every
                 // quartet in FP64 on the wave-per-quartet kernel
                 const bool lane_class = qf::fock_launcher(cb, ck) != nullptr;
-                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class;
+                // Only classes with La+Lb+Lc+Ld >= 5 are split: their FP64 per-lane
+                // kernels spill and the FP32 form halves the registers; for the
+                // lighter classes the second launch costs more than FP32 saves
+                // ((Gly)10 6-31G(d,p): (dp|pp) -1.2 ms, (dp|ds) -0.36 ms split, the
+                // Lsum <= 4 classes +0.04..+0.43 ms each).
+                const int Lsum_s = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
+                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= 5;
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
