@@ -651,10 +651,10 @@ def run_gly10(a, dev):
 
 
 def run_uhf(a, dev):
-    """(H2O)32+ doublet UHF/6-31G(d): the open-shell build (two passes of
-    the class kernels: G(Pa+Pb) and the exchange-only G(Pa-Pb)) on the
-    converged UHF densities, against the closed-shell build of the same
-    basis on Pa+Pb."""
+    """(H2O)32+ doublet UHF/6-31G(d): the open-shell build (one pass that
+    digests every integral with Pa+Pb and, for the exchange terms, with
+    Pa-Pb) on the converged UHF densities, against the closed-shell build
+    of the same basis on Pa+Pb."""
     import torch
     from quick_amd import basis as B
     from quick_amd import molecules as M

@@ -67,7 +67,7 @@ int qf_set_molecule(qf_engine *e, int natom, const double *xyz);
  *   (ksumtype: first basis function), exps[nprim_total] (gcexpo) and
  *   coef[nprim_total*4] the normalised coefficient of primitive p for L
  *   (QUICK's gccoeff of function ksumtype+L), at coef[p*4+L].
- *   Angular momentum must be <= 2. */
+ *   Angular momentum must be <= 3 (f); the ERI gradient covers shells up to d. */
 int qf_set_basis(qf_engine *e, int nshell, const int *shell_atom, const int *shell_lmin,
                  const int *shell_lmax, const int *shell_nprim, const int *shell_pstart,
                  const int *shell_fstart, int nprim_total, const double *exps, const double *coef,

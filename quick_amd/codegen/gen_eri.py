@@ -595,36 +595,54 @@ def launcher_source(name, diag, coop=False, lane=True):
         out += ["// Direct-atomic launch; FP64 only.  The FP32 instantiation of the direct",
                 "// kernel is not built (it gave wrong Fock matrices on gfx950, cause not",
                 "// found): a request for it is refused with a nonzero return.",
+                "// os: 1 = the fused open-shell pass (second exchange density)",
                 "int launch_fock_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int fp32,"
-                " int grid, hipStream_t s)" % name, "{",
+                " int os, int grid, hipStream_t s)" % name, "{",
                 "    if (fp32) return 1;",
                 "    if (nt <= 0) return 0;",
                 "    if (grid <= 0 || grid > nt) grid = nt;",
-                "    hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
+                "    if (os)",
+                "        hipLaunchKernelGGL((fock_kernel<%s, double, true>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);"
+                % name,
+                "    else",
+                "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
                 "    return 0;",
                 "}", "",
+                "// os: 0 closed shell, 1 open shell with a second LDS tile, 2 open shell",
+                "// with the second exchange terms straight to G2",
                 "void launch_fock_tile_%s(const TileGroup* g, int ng, const Task* t, const PairBlock* b,"
-                " const MolDesc* m, int fp32, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
+                " const MolDesc* m, int fp32, int os, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
                 "    if (ng <= 0) return;",
                 "    if (grid <= 0 || grid > ng) grid = ng;",
-                "    if (fp32)",
+                "    const int k2 = os == 1;",
+                "    if (fp32 && os)",
+                "        hipLaunchKernelGGL((fock_tile_kernel<%s, float, true>), dim3(grid), dim3(waves * WAVE), lds, s,"
+                " g, ng, t, b, m, k2);" % name,
+                "    else if (fp32)",
                 "        hipLaunchKernelGGL((fock_tile_kernel<%s, float>), dim3(grid), dim3(waves * WAVE), lds, s,"
-                " g, ng, t, b, m);" % name,
+                " g, ng, t, b, m, 0);" % name,
+                "    else if (os)",
+                "        hipLaunchKernelGGL((fock_tile_kernel<%s, double, true>), dim3(grid), dim3(waves * WAVE), lds, s,"
+                " g, ng, t, b, m, k2);" % name,
                 "    else",
                 "        hipLaunchKernelGGL((fock_tile_kernel<%s, double>), dim3(grid), dim3(waves * WAVE), lds, s,"
-                " g, ng, t, b, m);" % name,
+                " g, ng, t, b, m, 0);" % name,
                 "}", ""]
     if coop:
         out += ["// gscratch: COOP_GLOBAL_SLICES slices of LDS doubles when the class keeps its",
                 "// slots in global memory (the grid is then capped to that many workgroups)",
-                "void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int grid,"
-                " double* gscratch, hipStream_t s)" % name, "{",
+                "void launch_fock_coop_%s(const Task* t, int nt, const PairBlock* b, const MolDesc* m, int os,"
+                " int grid, double* gscratch, hipStream_t s)" % name, "{",
                 "    using D = %sCoop;" % name,
                 "    if (nt <= 0) return;",
                 "    if (grid <= 0 || grid > nt) grid = nt;",
                 "    if (D::GLOBAL_SLOTS && grid > COOP_GLOBAL_SLICES) grid = COOP_GLOBAL_SLICES;",
-                "    hipLaunchKernelGGL((fock_coop_kernel<D>), dim3(grid), dim3(D::NT),"
-                " D::GLOBAL_SLOTS ? 0 : D::LDS * sizeof(double), s, t, nt, b, m, gscratch);",
+                "    const size_t lds = D::GLOBAL_SLOTS ? 0 : D::LDS * sizeof(double);",
+                "    if (os)",
+                "        hipLaunchKernelGGL((fock_coop_kernel<D, true>), dim3(grid), dim3(D::NT), lds, s, t, nt, b, m,"
+                " gscratch);",
+                "    else",
+                "        hipLaunchKernelGGL((fock_coop_kernel<D>), dim3(grid), dim3(D::NT), lds, s, t, nt, b, m, gscratch);",
                 "}", ""]
     if diag:
         out += ["void launch_schwarz_%s(PairBlock blk, double* out, double* gscratch, hipStream_t s)" % name, "{",
@@ -682,20 +700,23 @@ def main():
     lines = ["// Generated by quick_amd/codegen/gen_eri.py - do not edit.", "#pragma once",
              "#include <hip/hip_runtime.h>", '#include "../qf_types.h"', "", "namespace qf {", "",
              "// returns nonzero when the launch is refused (FP32 on the direct kernel)",
-             "using FockLaunch = int (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);",
+             "// (tasks, ntasks, blocks, mols, fp32, os, grid, stream)",
+             "using FockLaunch = int (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, int, hipStream_t);",
+             "// (groups, ngroups, tasks, blocks, mols, fp32, os, waves, lds bytes, grid, stream)",
              "using FockTileLaunch = void (*)(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
-             " int, int, size_t, int, hipStream_t);",
-             "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, double*,"
+             " int, int, int, size_t, int, hipStream_t);",
+             "// (tasks, ntasks, blocks, mols, os, grid, global slots, stream)",
+             "using FockCoopLaunch = void (*)(const Task*, int, const PairBlock*, const MolDesc*, int, int, double*,"
              " hipStream_t);",
              "using SchwarzLaunch = void (*)(PairBlock, double*, double*, hipStream_t);", ""]
     for (name, La, Lb, Lc, Ld, diag) in names:
         if has_lane_kernels(La, Lb, Lc, Ld):
-            lines.append("int launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, hipStream_t);"
-                         % name)
+            lines.append("int launch_fock_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, int,"
+                         " hipStream_t);" % name)
             lines.append("void launch_fock_tile_%s(const TileGroup*, int, const Task*, const PairBlock*, const MolDesc*,"
-                         " int, int, size_t, int, hipStream_t);" % name)
+                         " int, int, int, size_t, int, hipStream_t);" % name)
         if (La, Lb, Lc, Ld) in COOP_CLASSES:
-            lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, double*,"
+            lines.append("void launch_fock_coop_%s(const Task*, int, const PairBlock*, const MolDesc*, int, int, double*,"
                          " hipStream_t);" % name)
         if diag:
             lines.append("void launch_schwarz_%s(PairBlock, double*, double*, hipStream_t);" % name)

@@ -51,6 +51,7 @@ const char* const kDirectFp32Refused =
     "built); FP32 halves run on the LDS-tile kernel";
 constexpr int kCoopWindow = 8;   // kets per task of the wave-per-quartet kernels
 constexpr int kNPC = 10;   // pair classes (La >= Lb) for s, p, d, f
+constexpr size_t kMaxTileLds = 64 * 1024;   // dynamic LDS of one tile workgroup
 const int kClassL[kNPC][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
 inline int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
@@ -149,9 +150,9 @@ __global__ void k_zero(double* __restrict__ p, size_t count)
 }
 
 __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* __restrict__ Gs,
-                           double* __restrict__ F, double* __restrict__ F2, double sign2, int n)
+                           double* __restrict__ F, int n)
 {
-    // G = sum over copies of (Gt + Gt^T); F += G, F2 += sign2 G
+    // G = sum over copies of (Gt + Gt^T); F += G
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
     const int i = idx / n, j = idx % n;
@@ -160,7 +161,25 @@ __global__ void k_finalize(const double* __restrict__ Gt, int replicas, double* 
     for (int r = 0; r < replicas; ++r) g += Gt[r * n2 + i * n + j] + Gt[r * n2 + j * n + i];
     Gs[idx] = g;
     if (F) F[idx] += g;
-    if (F2) F2[idx] += sign2 * g;
+}
+
+// Open-shell pass: A = sum(Gt + Gt^T) (J and exchange of Pt), B = sum(Gt2 +
+// Gt2^T) (exchange of Ps); Fa += A + B, Fb += A - B, Gs = A.
+__global__ void k_finalize_os(const double* __restrict__ Gt, const double* __restrict__ Gt2, int replicas,
+                              double* __restrict__ Gs, double* __restrict__ Fa, double* __restrict__ Fb, int n)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int i = idx / n, j = idx % n;
+    const size_t n2 = (size_t)n * n;
+    double a = 0.0, b = 0.0;
+    for (int r = 0; r < replicas; ++r) {
+        a += Gt[r * n2 + i * n + j] + Gt[r * n2 + j * n + i];
+        b += Gt2[r * n2 + i * n + j] + Gt2[r * n2 + j * n + i];
+    }
+    Gs[idx] = a;
+    Fa[idx] += a + b;
+    Fb[idx] += a - b;
 }
 
 // Counts, per quartet class, the quartets and primitive quartets that pass
@@ -257,6 +276,7 @@ struct qf_engine {
         int cb, ck, fp32, offset, count;
         int tile, goffset, gcount, waves;
         size_t lds;
+        size_t lds_os;   // open-shell pass with a second exchange tile (0: it does not fit)
     };
     std::vector<Launch> launches;
     bool prepared = false;
@@ -276,7 +296,7 @@ struct qf_engine {
     DevBuf<int> d_colf;
     DevBuf<double> d_Gt, d_Gs, d_cm, d_P;
     DevBuf<double> d_Pb, d_Pt, d_Ps, d_Fa, d_Fb;   // open-shell builds (allocated on first use)
-    DevBuf<double> d_zero;                          // nbasis^2 zeros: Coulomb density of the exchange-only pass
+    DevBuf<double> d_Gt2;                           // their second exchange accumulators (replicas x n^2)
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
@@ -716,7 +736,7 @@ int prepare(qf_engine* e)
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
                 const int win = coop ? kCoopWindow : qf::WAVE;
-                size_t lds = 0;
+                size_t lds = 0, lds_os = 0;
                 long long group_tasks = 0;
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {
                     const HostBlock& B = e->blocks[bbk];
@@ -791,9 +811,11 @@ int prepare(qf_engine* e)
                                 e->groups.push_back(tg);
                                 group_tasks += tg.task_end - tg.task_begin;
                             }
-                            if (chunk_end > chunk_first)
-                                lds = std::max(lds, sizeof(double) * ((size_t)(na + M * nb) * Kb.colf.size() +
-                                                                      (size_t)M * na * nb));
+                            if (chunk_end > chunk_first) {
+                                const size_t kt = (size_t)(na + M * nb) * Kb.colf.size();
+                                lds = std::max(lds, sizeof(double) * (kt + (size_t)M * na * nb));
+                                lds_os = std::max(lds_os, sizeof(double) * (2 * kt + (size_t)M * na * nb));
+                            }
                         }
                     }
                 }
@@ -846,6 +868,7 @@ int prepare(qf_engine* e)
                                                                   std::max<long long>(1, (group_tasks + gcount - 1) / gcount))
                                          : 1;
                     L.lds = lds;
+                    L.lds_os = lds_os <= kMaxTileLds ? lds_os : 0;
                     e->launches.push_back(L);
                 }
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
@@ -862,6 +885,8 @@ int prepare(qf_engine* e)
             const size_t need = sizeof(double) * ((size_t)(na + g.nbra * nb) * e->blocks[t0.ket_block].colf.size() +
                                                   (size_t)g.nbra * na * nb);
             if (need > L.lds) return fail(QF_ERR_STATE, "tile group exceeds its LDS allocation");
+            if (L.lds_os && 2 * need - sizeof(double) * (size_t)g.nbra * na * nb > L.lds_os)
+                return fail(QF_ERR_STATE, "tile group exceeds its open-shell LDS allocation");
             for (int x = g.task_begin; x < g.task_end; ++x) {
                 const qf::Task& t = e->tasks[x];
                 if (t.bra_block != t0.bra_block || t.ket_block != t0.ket_block || t.kb < g.chunk_kb ||
@@ -948,8 +973,8 @@ double* noalloc_scratch_ptr(int device)
     return g_scratch_ptr[device];
 }
 
-// One pass over every class launch: G(P) with the density screen dcm,
-// Coulomb terms on or off, then F += G and F2 += sign2 G.  `slot` picks the
+// One pass over every class launch: G(P) with the density screen dcm, then
+// F += G.  `slot` picks the
 // device MolDesc, so two passes can be queued back to back on one stream.
 // Where a build keeps G: `replicas` copies of nbasis^2 at Gt (the atomics'
 // targets) and the symmetrised G at Gs.  The engine's own buffers, or the
@@ -959,11 +984,17 @@ struct GBufs {
     int replicas;
     double* Gs;
     bool timing;   // per-launch HIP events (never in the no-alloc path: it creates events)
+    double* Gt2 = nullptr;   // open-shell pass: accumulators of the second exchange density
 };
 
-int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, double* dF, double* dF2, double sign2,
-               int slot, hipStream_t s, const GBufs& gb)
+// dP2 != nullptr: the fused open-shell pass.  Every integral is digested
+// with dP (Coulomb and exchange, into Gt) and with dP2 (exchange only, into
+// gb.Gt2); then dF += A + B and dF2 += A - B (k_finalize_os).
+int build_pass(qf_engine* e, const double* dP, const double* dcm, double* dF, int slot, hipStream_t s,
+               const GBufs& gb, const double* dP2 = nullptr, double* dF2 = nullptr)
 {
+    const int os = dP2 != nullptr;
+    if (os && (!gb.Gt2 || !dF || !dF2)) return fail(QF_ERR_ARG, "open-shell pass without its buffers");
     const int n = e->nbasis, ns = e->nshell;
     qf::MolDesc md{};
     md.P = dP;
@@ -976,11 +1007,13 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
     md.hyb = e->hyb;
     md.replicas = gb.replicas;
     md.split = (e->prec == QF_PREC_MIXED) ? e->split_thr : 0.0;
-    md.PJ = jon ? dP : e->d_zero.p;
+    md.P2 = dP2;
+    md.G2 = gb.Gt2;
     e->mol[slot] = md;
     qf::MolDesc* dmol = e->d_mol.p + slot;
     QF_HIP(hipMemcpyAsync(dmol, &e->mol[slot], sizeof(md), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, gb.Gt, (size_t)n * n * gb.replicas);
+    if (os) hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, gb.Gt2, (size_t)n * n * gb.replicas);
     for (size_t li = 0; li < e->launches.size(); ++li) {
         const auto& L = e->launches[li];
         hipEvent_t ea = nullptr, eb = nullptr;
@@ -999,22 +1032,28 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, int jon, doubl
         if (L.tile == 2) {
             qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
             if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, e->persist, e->d_cscratch.p, s);
+            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, os, e->persist, e->d_cscratch.p, s);
         } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
-            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, dmol, L.fp32, L.waves, L.lds,
-               e->persist, s);
+            // open shell: a second exchange tile where two fit, else its terms go to Gt2
+            const int tos = !os ? 0 : L.lds_os ? 1 : 2;
+            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, dmol, L.fp32, tos, L.waves,
+               tos == 1 ? L.lds_os : L.lds, e->persist, s);
         } else {
             qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
             if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
-            if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, e->persist, s))
+            if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, os, e->persist, s))
                 return fail(QF_ERR_ARG, kDirectFp32Refused);
         }
         if (gb.timing) QF_HIP(hipEventRecord(eb, s));
     }
-    hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, gb.Gt, gb.replicas, gb.Gs,
-                       dF, dF2, sign2, n);
+    if (os)
+        hipLaunchKernelGGL(k_finalize_os, dim3((n * n + 255) / 256), dim3(256), 0, s, gb.Gt, gb.Gt2, gb.replicas,
+                           gb.Gs, dF, dF2, n);
+    else
+        hipLaunchKernelGGL(k_finalize, dim3((n * n + 255) / 256), dim3(256), 0, s, gb.Gt, gb.replicas, gb.Gs,
+                           dF, n);
     QF_HIP(hipGetLastError());
     return QF_OK;
 }
@@ -1029,7 +1068,7 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
                            e->d_f1.p, e->d_cm.p);
         dcm = e->d_cm.p;
     }
-    const int rc = build_pass(e, dP, dcm, 1, dF, nullptr, 0.0, 0, s, GBufs{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing});
+    const int rc = build_pass(e, dP, dcm, dF, 0, s, GBufs{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing});
     if (rc == QF_OK) {
         e->built = true;
         e->count_cm_own = (dcm == e->d_cm.p);
@@ -1039,11 +1078,13 @@ int build(qf_engine* e, const double* dP, double* dF, const double* dcm, hipStre
 
 // Open-shell build (QUICK's get_oshell_eri, gpu_get2e_subs.h OSHELL
 // branches): Fa += J(Pt) - hyb K(Pa), Fb += J(Pt) - hyb K(Pb).  With the
-// spin density Ps = Pa - Pb this is two passes of the closed-shell kernels:
+// spin density Ps = Pa - Pb:
 //   A = J(Pt) - hyb/2 K(Pt)   (the closed-shell G of Pt)
 //   B = -hyb/2 K(Ps)          (exchange terms only)
-// and Fa += A + B, Fb += A - B.  Both passes screen with QUICK's open-shell
-// density matrix, so they see exactly the quartets QUICK's kernel sees.
+// and Fa += A + B, Fb += A - B.  One pass computes each integral once and
+// digests it with both densities (the OS kernel instantiations); it screens
+// with QUICK's open-shell density matrix, so it sees exactly the quartets
+// QUICK's kernel sees.
 int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, double* dFb, const double* dcm,
              hipStream_t s)
 {
@@ -1051,10 +1092,6 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
     const int n = e->nbasis, ns = e->nshell, n2 = n * n;
     QF_HIP(e->d_Pt.resize(n2));
     QF_HIP(e->d_Ps.resize(n2));
-    if (e->d_zero.n < (size_t)n2) {
-        QF_HIP(e->d_zero.resize(n2));
-        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, e->d_zero.p, (size_t)n2);
-    }
     hipLaunchKernelGGL(k_spin, dim3((n2 + 255) / 256), dim3(256), 0, s, dPa, dPb, e->d_Pt.p, e->d_Ps.p, n2);
     if (!dcm) {
         const int nthr = 256, tot = ns * ns;
@@ -1062,10 +1099,10 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
                            e->d_f1.p, e->d_cm.p);
         dcm = e->d_cm.p;
     }
-    const GBufs gb{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing};
-    int rc = build_pass(e, e->d_Pt.p, dcm, 1, dFa, dFb, 1.0, 0, s, gb);
-    if (rc) return rc;
-    rc = build_pass(e, e->d_Ps.p, dcm, 0, dFa, dFb, -1.0, 1, s, gb);
+    QF_HIP(e->d_Gt2.resize((size_t)n2 * e->replicas));
+    GBufs gb{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing};
+    gb.Gt2 = e->d_Gt2.p;
+    const int rc = build_pass(e, e->d_Pt.p, dcm, dFa, 0, s, gb, e->d_Ps.p, dFb);
     if (rc == QF_OK) {
         e->built = true;
         e->count_cm_own = (dcm == e->d_cm.p);
@@ -1238,7 +1275,7 @@ int qf_fock_build_device_noalloc(qf_engine* e, const double* d_P, double* d_F, c
                            e->d_f1.p, cm);
         d_cutmat = cm;
     }
-    return build_pass(e, d_P, d_cutmat, 1, d_F, nullptr, 0.0, 2, s, GBufs{Gt, 1, Gs, false});
+    return build_pass(e, d_P, d_cutmat, d_F, 2, s, GBufs{Gt, 1, Gs, false});
 }
 
 // Two-electron gradient (QUICK's gpu_get_cshell_eri_grad_ -> getGrad,
@@ -1261,7 +1298,6 @@ int grad_pass(qf_engine* e, const double* dP, const double* dcm, double* dgrad, 
         QF_HIP(e->d_gscratch.resize(gdoubles * qf::COOP_GLOBAL_SLICES));
     qf::MolDesc md{};
     md.P = dP;
-    md.PJ = dP;
     md.G = nullptr;
     md.cutmat = dcm;
     md.nbasis = e->nbasis;
@@ -1752,16 +1788,16 @@ int batch_issue(qf_batch* b, hipStream_t s)
         if (L.tile == 2) {
             qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
             if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, b->persist, b->d_cscratch.p, s);
+            fc(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, 0, b->persist, b->d_cscratch.p, s);
         } else if (L.tile) {
             qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
             if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
-            ft(b->d_groups.p + L.goffset, L.gcount, b->d_tasks.p, b->d_blocks.p, b->d_moldesc.p, L.fp32, L.waves,
+            ft(b->d_groups.p + L.goffset, L.gcount, b->d_tasks.p, b->d_blocks.p, b->d_moldesc.p, L.fp32, 0, L.waves,
                L.lds, b->persist, s);
         } else {
             qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
             if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
-            if (f(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, L.fp32, b->persist, s))
+            if (f(b->d_tasks.p + L.offset, L.count, b->d_blocks.p, b->d_moldesc.p, L.fp32, 0, b->persist, s))
                 return fail(QF_ERR_ARG, kDirectFp32Refused);
         }
         if (b->timing) QF_HIP(hipEventRecord(eb, s));
@@ -1836,7 +1872,6 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         if (d_P && (!d_P[i] || !d_F[i])) return bail(fail(QF_ERR_ARG, "NULL density/Fock pointer"));
         qf::MolDesc m{};
         m.P = d_P ? d_P[i] : b->d_P.p + poff[i];
-        m.PJ = m.P;
         m.G = b->d_Gt.p + gtoff[i];
         m.cutmat = b->d_cm.p + cmoff[i];
         m.nbasis = e->nbasis;

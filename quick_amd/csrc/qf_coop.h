@@ -186,11 +186,14 @@ __device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki,
     }
 }
 
-template <class D, typename SP>
+// OS (the fused open-shell pass): after the six blocks of md.P the
+// four exchange blocks are reloaded from md.P2 and the exchange terms are
+// contracted a second time into G2.
+template <class D, bool OS, typename SP>
 __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb, int i, const PairInfo& bi,
                                              const PairBlock& kb, int j, const PairInfo& ki, double dn,
-                                             const MolDesc& md, QF_AS_GLOBAL double* G, SP s,
-                                             const QF_AS_LDS double* boys_tab)
+                                             const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_GLOBAL double* G2,
+                                             SP s, const QF_AS_LDS double* boys_tab)
 {
     constexpr int na = ncart(D::La), nb = ncart(D::Lb), nc = ncart(D::Lc), nd = ncart(D::Ld);
     const int lane = threadIdx.x;
@@ -213,9 +216,8 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
     SP Dad = Dac + na * nc;
     SP Dbc = Dad + na * nd;
     SP Dbd = Dbc + nb * nc;
-    const auto PJ = gptr(md.PJ);              // Coulomb density (zeros in an exchange-only build)
-    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = PJ[(bi.fa + x / nb) * n + bi.fb + x % nb];
-    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = PJ[(ki.fa + x / nd) * n + ki.fb + x % nd];
+    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = P[(bi.fa + x / nb) * n + bi.fb + x % nb];
+    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = P[(ki.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < na * nc; x += D::NT) Dac[x] = P[(bi.fa + x / nc) * n + ki.fa + x % nc];
     for (int x = lane; x < na * nd; x += D::NT) Dad[x] = P[(bi.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < nb * nc; x += D::NT) Dbc[x] = P[(bi.fb + x / nc) * n + ki.fa + x % nc];
@@ -273,6 +275,46 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
         }
         if (v != 0.0) atomic_add(&G[row * n + col], v);
     }
+    if constexpr (OS) {
+        __syncthreads();
+        const auto P2 = gptr(md.P2);
+        for (int x = lane; x < na * nc; x += D::NT) Dac[x] = P2[(bi.fa + x / nc) * n + ki.fa + x % nc];
+        for (int x = lane; x < na * nd; x += D::NT) Dad[x] = P2[(bi.fa + x / nd) * n + ki.fb + x % nd];
+        for (int x = lane; x < nb * nc; x += D::NT) Dbc[x] = P2[(bi.fb + x / nc) * n + ki.fa + x % nc];
+        for (int x = lane; x < nb * nd; x += D::NT) Dbd[x] = P2[(bi.fb + x / nd) * n + ki.fb + x % nd];
+        __syncthreads();
+        for (int o = o2 + lane; o < o6; o += D::NT) {
+            double v = 0.0;
+            int row, col;
+            if (o < o3) {
+                const int a = (o - o2) / nc, c = (o - o2) % nc;
+                for (int b = 0; b < nb; ++b)
+                    for (int d = 0; d < nd; ++d) v += Dbd[b * nd + d] * IX(a, b, c, d);
+                row = bi.fa + a;
+                col = ki.fa + c;
+            } else if (o < o4) {
+                const int a = (o - o3) / nd, d = (o - o3) % nd;
+                for (int b = 0; b < nb; ++b)
+                    for (int c = 0; c < nc; ++c) v += Dbc[b * nc + c] * IX(a, b, c, d);
+                row = bi.fa + a;
+                col = ki.fb + d;
+            } else if (o < o5) {
+                const int b = (o - o4) / nc, c = (o - o4) % nc;
+                for (int a = 0; a < na; ++a)
+                    for (int d = 0; d < nd; ++d) v += Dad[a * nd + d] * IX(a, b, c, d);
+                row = bi.fb + b;
+                col = ki.fa + c;
+            } else {
+                const int b = (o - o5) / nd, d = (o - o5) % nd;
+                for (int a = 0; a < na; ++a)
+                    for (int c = 0; c < nc; ++c) v += Dac[a * nc + c] * IX(a, b, c, d);
+                row = bi.fb + b;
+                col = ki.fb + d;
+            }
+            v *= kscale;
+            if (v != 0.0) atomic_add(&G2[row * n + col], v);
+        }
+    }
 #undef IX
     __syncthreads();   // the next quartet reuses the slots
 }
@@ -294,7 +336,7 @@ __device__ __forceinline__ CoopSlots<D> coop_slots(double* gscratch)
 // screens the task's quartets lane-parallel, then processes the surviving
 // quartets one after another.  FP64 throughout.  Dynamic LDS: D::LDS doubles
 // (none with GLOBAL_SLOTS).
-template <class D>
+template <class D, bool OS = false>
 __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict__ tasks, int ntasks,
                                                          const PairBlock* __restrict__ blocks,
                                                          const MolDesc* __restrict__ mols, double* gscratch)
@@ -310,7 +352,9 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
         const PairBlock kb = cptr(blocks)[t.ket_block];
         const MolDesc md = cptr(mols)[bb.mol];
         const int ns = md.nshell;
-        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis);
+        const size_t roff = (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis;
+        const auto G = gptr_rw(md.G + roff);
+        const auto G2 = gptr_rw(OS ? md.G2 + roff : md.G);
         const auto cm = gptr(md.cutmat);
         const auto bperm = cptr(bb.bperm);
         const bool in_range = lane < t.ket_count;
@@ -334,7 +378,7 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
                 const int j = t.ket_start + v;
                 const PairInfo ki = cptr(kb.info)[j];
                 const double dnv = __shfl(dn, v, WAVE);
-                coop_quartet<D>(t, bb, i, bi, kb, j, ki, dnv, md, G, s, QF_BOYS_LDS_PTR);
+                coop_quartet<D, OS>(t, bb, i, bi, kb, j, ki, dnv, md, G, G2, s, QF_BOYS_LDS_PTR);
             }
         }
     }

@@ -325,8 +325,7 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 //   Gt_ac -= s/4 hyb D_bd I, Gt_ad -= .. D_bc I, Gt_bc -= .. D_ad I, Gt_bd -= .. D_ac I,
 // and G = Gt + Gt^T afterwards.  Because a is fixed for the wave, J_cd, K_ac
 // and K_ad stay in registers across the bra loop and J_ab is summed over
-// the wave per bra.  The Coulomb terms read their density from
-// md.PJ: P itself, or zeros in the exchange-only pass of the open-shell build.
+// the wave per bra.
 //
 // TILE = false: everything goes straight to G with FP64 atomics (K_bc and
 // K_bd per quartet).  TILE = true: J_ab and every exchange term go to the
@@ -338,14 +337,18 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
 // ---------------------------------------------------------------------------
 struct TileRef {
     QF_AS_LDS double* k;     // nrow x ncol exchange tile
+    QF_AS_LDS double* k2;    // the same for the second exchange density (open-shell pass)
     QF_AS_LDS double* jab;   // nbra x na x nb Coulomb terms of the bras
     int ncol, chunk_kb;
     const QF_AS_LDS double* boys;   // Boys fit table (kBoysFit) in LDS
 };
 
-template <class C, typename R, bool TILE>
+// OS = true (the fused open-shell pass): every integral is digested a second
+// time into exchange terms of md.P2 that go to G2 (or the tile k2).
+template <class C, typename R, bool TILE, bool OS>
 __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, const PairBlock& kb,
-                                          const MolDesc& md, QF_AS_GLOBAL double* G, const TileRef& tr)
+                                          const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_GLOBAL double* G2,
+                                          const TileRef& tr)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb), nc = ncart(C::Lc), nd = ncart(C::Ld);
     constexpr bool kFP32 = sizeof(R) == 4;
@@ -356,8 +359,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     const int n = md.nbasis, ns = md.nshell;
     const auto cmc = cptr(md.cutmat);
     const auto cm = gptr(md.cutmat);
-    const auto PJc = cptr(md.PJ);
-    const auto PJ = gptr(md.PJ);
+    const auto Pc = cptr(md.P);
     const auto P = gptr(md.P);
     const auto bperm = cptr(bb.bperm);
 
@@ -368,25 +370,41 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     const double cm_ac = cm[sa * ns + ki.sa];
     const double cm_ad = cm[sa * ns + ki.sb];
 
+    const auto P2 = gptr(md.P2);
+    constexpr int os2 = OS ? 1 : 0;          // sizes of the second-density arrays (1 when unused)
     double Dcd[nc * nd], Dac[na * nc], Dad[na * nd];
+    double Dac2[os2 * na * nc + 1 - os2], Dad2[os2 * na * nd + 1 - os2];
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = PJ[(ki.fa + c) * n + ki.fb + d];
+        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = P[(ki.fa + c) * n + ki.fb + d];
 #pragma unroll
     for (int a = 0; a < na; ++a) {
 #pragma unroll
         for (int c = 0; c < nc; ++c) Dac[a * nc + c] = P[(fa + a) * n + ki.fa + c];
 #pragma unroll
         for (int d = 0; d < nd; ++d) Dad[a * nd + d] = P[(fa + a) * n + ki.fb + d];
+        if constexpr (OS) {
+#pragma unroll
+            for (int c = 0; c < nc; ++c) Dac2[a * nc + c] = P2[(fa + a) * n + ki.fa + c];
+#pragma unroll
+            for (int d = 0; d < nd; ++d) Dad2[a * nd + d] = P2[(fa + a) * n + ki.fb + d];
+        }
     }
     double Jcd[nc * nd], Kac[na * nc], Kad[na * nd];
+    double Kac2[os2 * na * nc + 1 - os2], Kad2[os2 * na * nd + 1 - os2];
 #pragma unroll
     for (int k = 0; k < nc * nd; ++k) Jcd[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < na * nc; ++k) Kac[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < na * nd; ++k) Kad[k] = 0.0;
+    if constexpr (OS) {
+#pragma unroll
+        for (int k = 0; k < na * nc; ++k) Kac2[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < na * nd; ++k) Kad2[k] = 0.0;
+    }
     bool touched = false;
     const double kscale = -0.25 * md.hyb;
 
@@ -423,7 +441,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
-            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = PJc[(bi.fa + a) * n + bi.fb + b];
+            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = Pc[(bi.fa + a) * n + bi.fb + b];
 #pragma unroll
         for (int b = 0; b < nb; ++b) {
 #pragma unroll
@@ -432,12 +450,27 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int d = 0; d < nd; ++d) Dbd[b * nd + d] = P[(bi.fb + b) * n + ki.fb + d];
         }
         double Jab[na * nb], Kbc[nb * nc], Kbd[nb * nd];
+        double Dbc2[os2 * nb * nc + 1 - os2], Dbd2[os2 * nb * nd + 1 - os2];
+        double Kbc2[os2 * nb * nc + 1 - os2], Kbd2[os2 * nb * nd + 1 - os2];
 #pragma unroll
         for (int k = 0; k < na * nb; ++k) Jab[k] = 0.0;
 #pragma unroll
         for (int k = 0; k < nb * nc; ++k) Kbc[k] = 0.0;
 #pragma unroll
         for (int k = 0; k < nb * nd; ++k) Kbd[k] = 0.0;
+        if constexpr (OS) {
+#pragma unroll
+            for (int b = 0; b < nb; ++b) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) Dbc2[b * nc + c] = P2[(bi.fb + b) * n + ki.fa + c];
+#pragma unroll
+                for (int d = 0; d < nd; ++d) Dbd2[b * nd + d] = P2[(bi.fb + b) * n + ki.fb + d];
+            }
+#pragma unroll
+            for (int k = 0; k < nb * nc; ++k) Kbc2[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < nb * nd; ++k) Kbd2[k] = 0.0;
+        }
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
@@ -454,6 +487,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         Kad[a * nd + d] += Dbc[b * nc + c] * v;
                         Kbc[b * nc + c] += Dad[a * nd + d] * v;
                         Kbd[b * nd + d] += Dac[a * nc + c] * v;
+                        if constexpr (OS) {
+                            Kac2[a * nc + c] += Dbd2[b * nd + d] * v;
+                            Kad2[a * nd + d] += Dbc2[b * nc + c] * v;
+                            Kbc2[b * nc + c] += Dad2[a * nd + d] * v;
+                            Kbd2[b * nd + d] += Dac2[a * nc + c] * v;
+                        }
                     }
         // bra block: summed over the wave
 #pragma unroll
@@ -478,6 +517,22 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                     for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kbc[b * nc + c]);
 #pragma unroll
                     for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kbd[b * nd + d]);
+                    if constexpr (OS) {
+                        if (tr.k2) {
+                            QF_AS_LDS double* row2 = tr.k2 + (r0 + b) * tr.ncol;
+#pragma unroll
+                            for (int c = 0; c < nc; ++c) lds_add(&row2[ki.cola + c], Kbc2[b * nc + c]);
+#pragma unroll
+                            for (int d = 0; d < nd; ++d) lds_add(&row2[ki.colb + d], Kbd2[b * nd + d]);
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < nc; ++c)
+                                atomic_add(&G2[(bi.fb + b) * n + ki.fa + c], kscale * Kbc2[b * nc + c]);
+#pragma unroll
+                            for (int d = 0; d < nd; ++d)
+                                atomic_add(&G2[(bi.fb + b) * n + ki.fb + d], kscale * Kbd2[b * nd + d]);
+                        }
+                    }
                 }
             } else {
 #pragma unroll
@@ -488,6 +543,14 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
                     for (int d = 0; d < nd; ++d)
                         atomic_add(&G[(bi.fb + b) * n + ki.fb + d], kscale * Kbd[b * nd + d]);
+                    if constexpr (OS) {
+#pragma unroll
+                        for (int c = 0; c < nc; ++c)
+                            atomic_add(&G2[(bi.fb + b) * n + ki.fa + c], kscale * Kbc2[b * nc + c]);
+#pragma unroll
+                        for (int d = 0; d < nd; ++d)
+                            atomic_add(&G2[(bi.fb + b) * n + ki.fb + d], kscale * Kbd2[b * nd + d]);
+                    }
                 }
             }
         }
@@ -505,6 +568,22 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kac[a * nc + c]);
 #pragma unroll
             for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kad[a * nd + d]);
+            if constexpr (OS) {
+                if (tr.k2) {
+                    QF_AS_LDS double* row2 = tr.k2 + a * tr.ncol;
+#pragma unroll
+                    for (int c = 0; c < nc; ++c) lds_add(&row2[ki.cola + c], Kac2[a * nc + c]);
+#pragma unroll
+                    for (int d = 0; d < nd; ++d) lds_add(&row2[ki.colb + d], Kad2[a * nd + d]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < nc; ++c)
+                        atomic_add(&G2[(fa + a) * n + ki.fa + c], kscale * Kac2[a * nc + c]);
+#pragma unroll
+                    for (int d = 0; d < nd; ++d)
+                        atomic_add(&G2[(fa + a) * n + ki.fb + d], kscale * Kad2[a * nd + d]);
+                }
+            }
         }
     } else {
 #pragma unroll
@@ -513,6 +592,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int c = 0; c < nc; ++c) atomic_add(&G[(fa + a) * n + ki.fa + c], kscale * Kac[a * nc + c]);
 #pragma unroll
             for (int d = 0; d < nd; ++d) atomic_add(&G[(fa + a) * n + ki.fb + d], kscale * Kad[a * nd + d]);
+            if constexpr (OS) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) atomic_add(&G2[(fa + a) * n + ki.fa + c], kscale * Kac2[a * nc + c]);
+#pragma unroll
+                for (int d = 0; d < nd; ++d) atomic_add(&G2[(fa + a) * n + ki.fb + d], kscale * Kad2[a * nd + d]);
+            }
         }
     }
 }
@@ -520,7 +605,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 // One wave per task, every term straight to G (replica blockIdx % replicas).
 // The grid may be smaller than the task count: each workgroup then strides
 // over the tasks (a persistent launch).
-template <class C, typename R>
+template <class C, typename R, bool OS = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
 void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __restrict__ blocks,
                  const MolDesc* __restrict__ mols)
@@ -535,8 +620,10 @@ void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __
         const PairBlock kb = cptr(blocks)[t.ket_block];
         const MolDesc md = cptr(mols)[bb.mol];
         // spread the atomics of concurrent waves over several copies of G
-        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis);
-        fock_task<C, R, false>(t, bb, kb, md, G, tr);
+        const size_t roff = (size_t)(blockIdx.x % md.replicas) * md.nbasis * md.nbasis;
+        const auto G = gptr_rw(md.G + roff);
+        const auto G2 = gptr_rw(OS ? md.G2 + roff : md.G);
+        fock_task<C, R, false, OS>(t, bb, kb, md, G, G2, tr);
     }
 }
 
@@ -544,11 +631,13 @@ void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __
 // smaller): its waves take the group's tasks in turn and accumulate J_ab
 // and the exchange terms in LDS; after a barrier the workgroup adds the
 // non-zero tile elements to G.  Dynamic LDS:
-// (na + nbra*nb) * ncol + nbra*na*nb doubles.
-template <class C, typename R>
+// (na + nbra*nb) * ncol * (k2 ? 2 : 1) + nbra*na*nb doubles.  OS: the
+// second exchange terms go to a second LDS tile (k2 = 1) or straight to
+// G2 (k2 = 0, where two tiles do not fit).
+template <class C, typename R, bool OS = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(WAVE, 4 * WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
 void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const Task* __restrict__ tasks,
-                      const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols)
+                      const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols, int k2)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb);
     extern __shared__ double qf_lds[];
@@ -565,14 +654,18 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
         const int n = md.nbasis;
         const int nrow = na + g.nbra * nb, ncol = kb.ncol;
         const int ntile = nrow * ncol, njab = g.nbra * na * nb;
-        for (int x = threadIdx.x; x < ntile + njab; x += blockDim.x) lds[x] = 0.0;
+        const bool two = OS && k2;
+        const int nk = two ? 2 * ntile : ntile;
+        for (int x = threadIdx.x; x < nk + njab; x += blockDim.x) lds[x] = 0.0;
         __syncthreads();
 
-        const auto G = gptr_rw(md.G + (size_t)(blockIdx.x % md.replicas) * n * n);
-        const TileRef tr{lds, lds + ntile, ncol, g.chunk_kb, QF_BOYS_LDS_PTR};
+        const size_t roff = (size_t)(blockIdx.x % md.replicas) * n * n;
+        const auto G = gptr_rw(md.G + roff);
+        const auto G2 = gptr_rw(OS ? md.G2 + roff : md.G);
+        const TileRef tr{lds, two ? lds + ntile : nullptr, lds + nk, ncol, g.chunk_kb, QF_BOYS_LDS_PTR};
         for (int ti = g.task_begin + (int)(threadIdx.x / WAVE); ti < g.task_end; ti += nw) {
             const Task t = cptr(tasks)[ti];
-            fock_task<C, R, true>(t, bb, kb, md, G, tr);
+            fock_task<C, R, true, OS>(t, bb, kb, md, G, G2, tr);
         }
         __syncthreads();
 
@@ -581,10 +674,11 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
         const auto bperm = cptr(bb.bperm);
         const auto colf = gptr(kb.colf);
         const int fa = cptr(bb.info)[bperm[g.chunk_kb]].fa;
-        for (int x = threadIdx.x; x < ntile; x += blockDim.x) {
+        for (int x = threadIdx.x; x < nk; x += blockDim.x) {
             const double v = lds[x];
             if (v == 0.0) continue;
-            const int r = x / ncol, q = x - r * ncol;
+            const int xt = x < ntile ? x : x - ntile;
+            const int r = xt / ncol, q = xt - r * ncol;
             int fr;
             if (r < na) {
                 fr = fa + r;
@@ -592,10 +686,10 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
                 const int k = (r - na) / nb;
                 fr = gptr(bb.info)[bperm[g.chunk_kb + k]].fb + (r - na - k * nb);
             }
-            atomic_add(&G[fr * n + colf[q]], kscale * v);
+            atomic_add(&(x < ntile ? G : G2)[fr * n + colf[q]], kscale * v);
         }
         for (int x = threadIdx.x; x < njab; x += blockDim.x) {
-            const double v = lds[ntile + x];
+            const double v = lds[nk + x];
             if (v == 0.0) continue;
             const int k = x / (na * nb), ab = x - k * (na * nb);
             const PairInfo bi = gptr(bb.info)[bperm[g.chunk_kb + k]];

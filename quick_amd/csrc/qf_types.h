@@ -44,13 +44,14 @@ struct PairBlock {
 
 struct MolDesc {
     const double* P;        // nbasis x nbasis density, row-major
-    const double* PJ;       // density of the Coulomb terms: P, or zeros for an exchange-only build
     double* G;              // nbasis x nbasis accumulator (G = Gt + Gt^T afterwards)
     const double* cutmat;   // nshell x nshell max |P| per original shell pair
     int nbasis, nshell;
     double cutoff, primlim, hyb;
     double split;           // quartets with Ycut*Ycut >= split run in FP64, below in FP32
     int replicas;           // G has `replicas` copies of nbasis^2; a wave adds into one of them
+    const double* P2;       // open-shell pass: the second exchange density (P_alpha - P_beta)
+    double* G2;             // open-shell pass: its accumulator, replicas x nbasis^2 like G
     int natom;              // gradient: atoms (3 natom entries of grad)
     const int* shell_atom;  // gradient: original shell -> atom
     double* grad;           // gradient: 3 natom accumulator, x y z per atom
