@@ -41,6 +41,19 @@ __device__ __forceinline__ QF_AS_GLOBAL T* gptr_rw(T* p)
     return (QF_AS_GLOBAL T*)(p);
 }
 
+// A value that the caller knows to be the same in every active lane, moved
+// to a scalar register.  Loop counters of loops whose exit differs between
+// lanes are kept in vector registers by the compiler (it cannot tell that
+// the active lanes agree), and so would be the loads they index.
+__host__ __device__ __forceinline__ int wave_uniform(int x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(x);
+#else
+    return x;
+#endif
+}
+
 // 1/sqrt(x): the hardware reciprocal square root on the device, the C
 // library on the host (the host path only serves CPU checks of this code).
 __host__ __device__ __forceinline__ double qf_rsqrt(double x)
@@ -222,6 +235,36 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// v moved between lanes by the DPP control CTRL (rows in ROW_MASK only;
+// lanes without a source, or in other rows, get 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned in every lane (a scalar register).  All
+// lanes must be active.  DPP steps: within quads (lane ^ 1, lane ^ 2), row
+// shifts by 4 and 8 (lane 15 of each row holds the row's sum), broadcast of
+// lane 15 into the next row and of lane 31 into rows 2 and 3; lane 63 then
+// holds the total.  A fixed order, so the result is deterministic.
+__device__ __forceinline__ double wave_sum_all_active(double v)
+{
+    v += dpp_f64<0xb1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4e, 0xf>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_f64<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, WAVE - 1);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), WAVE - 1);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ void atomic_add(QF_AS_GLOBAL double* p, double v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -277,7 +320,9 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
         const double Kk = KP(PF_KC, ko);
         const double Qk[3] = {KP(PF_PX, ko), KP(PF_PY, ko), KP(PF_PZ, ko)};
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
-        for (int ip = pb; ip < pe; ++ip) {
+        for (int ip0 = pb; ip0 < pe; ++ip0) {
+            // the active lanes agree on ip: scalar loads of the bra data
+            const int ip = BRA_UNIFORM ? wave_uniform(ip0) : ip0;
             if (screen && !((dn * BP(PF_CUT, ip)) * cutk > primlim)) break;
             const double zb = BP(PF_ZETA, ip);
             const double ozb = BP(PF_OOZ2, ip);
@@ -408,7 +453,8 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     bool touched = false;
     const double kscale = -0.25 * md.hyb;
 
-    for (int kk = t.kb; kk < t.ke; ++kk) {
+    for (int kk0 = t.kb; kk0 < t.ke; ++kk0) {
+        const int kk = wave_uniform(kk0);
         const int i = bperm[kk];
         const PairInfo bi = cptr(bb.info)[i];
         const double ytest = bi.ycut * ki.ycut;
@@ -499,7 +545,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         for (int a = 0; a < na; ++a)
 #pragma unroll
             for (int b = 0; b < nb; ++b) {
-                const double v = wave_sum(Jab[a * nb + b]);
+                const double v = wave_sum_all_active(Jab[a * nb + b]);
                 if (lane == 0 && v != 0.0) {
                     if (TILE)
                         lds_add(&tr.jab[((kk - tr.chunk_kb) * na + a) * nb + b], v);
@@ -663,8 +709,12 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
         const auto G = gptr_rw(md.G + roff);
         const auto G2 = gptr_rw(OS ? md.G2 + roff : md.G);
         const TileRef tr{lds, two ? lds + ntile : nullptr, lds + nk, ncol, g.chunk_kb, QF_BOYS_LDS_PTR};
-        for (int ti = g.task_begin + (int)(threadIdx.x / WAVE); ti < g.task_end; ti += nw) {
-            const Task t = cptr(tasks)[ti];
+        // the wave index through readfirstlane: the compiler cannot prove
+        // threadIdx.x / 64 wave-uniform, and would otherwise keep the task,
+        // its loop bounds and the bra primitive data in vector registers
+        const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+        for (int ti = g.task_begin + wave; ti < g.task_end; ti += nw) {
+            const Task t = cptr(tasks)[wave_uniform(ti)];
             fock_task<C, R, true, OS>(t, bb, kb, md, G, G2, tr);
         }
         __syncthreads();

@@ -320,7 +320,8 @@ __device__ __forceinline__ void grad_lane_contract(const PairBlock& bb, int i, c
         const double gk = KP(PF_ALPHA, ko);
         const double Qk[3] = {KP(PF_PX, ko), KP(PF_PY, ko), KP(PF_PZ, ko)};
         const double QC[3] = {Qk[0] - ki.A[0], Qk[1] - ki.A[1], Qk[2] - ki.A[2]};
-        for (int ip = pb; ip < pe; ++ip) {
+        for (int ip0 = pb; ip0 < pe; ++ip0) {
+            const int ip = wave_uniform(ip0);    // the active lanes agree: scalar loads of the bra data
             if (!((dn * BP(PF_CUT, ip)) * cutk > primlim)) break;
             const double zb = BP(PF_ZETA, ip);
             const double ozb = BP(PF_OOZ2, ip);
