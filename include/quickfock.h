@@ -228,11 +228,11 @@ int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
 int qf_set_tile(qf_engine *e, int enable, int waves, int tasks_per_group, int lds_bytes);
 
 /* Per-class launch shape (native; used by the autotuner): for the quartet
- * class (cb|ck), pair classes 0..5 = ss, ps, pp, ds, dp, dd with ck <= cb:
- * tile (-1 engine default, 0 direct atomics, 1 LDS tile, 2 one quartet per
- * wave with the recursions in LDS - d-heavy classes; others use the tile),
- * waves per tile workgroup, tasks per tile workgroup and bras per task
- * (0: default). */
+ * class (cb|ck), pair classes 0..9 = ss, ps, pp, ds, dp, dd, fs, fp, fd, ff
+ * with ck <= cb: tile (-1 engine default, 0 direct atomics, 1 LDS tile, 2
+ * one quartet per wave with the recursions in LDS - d-heavy classes; others
+ * use the tile), waves per tile workgroup, tasks per tile workgroup (with
+ * tile = 2: kets per task, 1..64, default 8) and bras per task (0: default). */
 int qf_set_class_tuning(qf_engine *e, int cb, int ck, int tile, int waves, int tasks_per_group,
                         int bras_per_task);
 

@@ -16,12 +16,13 @@ import numpy as np
 
 # (digestion, waves, tasks_per_group, bras_per_task); digestion 0 direct
 # atomics, 1 LDS tile, 2 wave per quartet (classes without that kernel use
-# the tile); 0 elsewhere = engine default
+# the tile; for it tasks_per_group is the ket window); 0 elsewhere = engine
+# default
 CANDIDATES = [
     (0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 0, 2), (0, 0, 0, 4),
     (1, 4, 16, 0), (1, 4, 8, 0), (1, 2, 8, 0), (1, 1, 4, 0), (1, 4, 32, 0),
     (1, 4, 16, 2), (1, 4, 16, 4), (1, 4, 32, 2), (1, 4, 32, 4),
-    (2, 0, 0, 0), (2, 0, 0, 1), (2, 0, 0, 2),
+    (2, 0, 0, 0), (2, 0, 0, 1), (2, 0, 0, 2), (2, 0, 2, 1), (2, 0, 4, 1), (2, 0, 16, 1),
 ]
 
 

@@ -735,7 +735,11 @@ int prepare(qf_engine* e)
                 if (!coop && !lane_class) return fail(QF_ERR_STATE, "class has neither per-lane nor wave-per-quartet kernels");
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
                 if (tile) mmax = (int)std::min<long long>(mmax, mfit);
-                const int win = coop ? kCoopWindow : qf::WAVE;
+                // kets per task: a full wave for the per-lane kernels; for the
+                // wave-per-quartet kernel a window (the class tuning's
+                // tasks_per_group): small windows give more workgroups to
+                // classes with few quartets
+                const int win = coop ? (ct.tpg > 0 ? std::min(ct.tpg, qf::WAVE) : kCoopWindow) : qf::WAVE;
                 size_t lds = 0, lds_os = 0;
                 long long group_tasks = 0;
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {

@@ -42,6 +42,55 @@ struct CoopHNode {
 constexpr int kVU = 2;   // VRR nodes per lane in flight
 constexpr int kHU = 4;   // HRR nodes per lane in flight
 
+// Runs a level-scheduled node program: levels [lev[l], lev[l+1]) of nodes
+// `prog`, U nodes per lane per round, a barrier after each level.  The
+// rounds are one uniform sequence over all levels, and the descriptors of
+// the next round are loaded before the current round reads and writes its
+// slots, so their global-memory latency overlaps the LDS work and the
+// barriers.  exec(node, x_valid) evaluates one node.
+template <int U, int NT, typename Node, typename Exec>
+__device__ __forceinline__ void run_levels(const QF_AS_GLOBAL Node* prog, const QF_AS_CONST int* lev, int nlev,
+                                           Exec exec)
+{
+    const int lane = threadIdx.x;
+    if (nlev <= 0) return;
+    int lv = 0, e1 = lev[1], x0 = lev[0];
+    Node q[U], qn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int x = x0 + u * NT + lane;
+        q[u] = prog[x < e1 ? x : e1 - 1];
+    }
+    for (;;) {
+        int nx0 = x0 + U * NT, nlv = lv, ne1 = e1;
+        const bool level_end = nx0 >= e1;
+        if (level_end) {
+            nlv = lv + 1;
+            if (nlv < nlev) {
+                nx0 = lev[nlv];
+                ne1 = lev[nlv + 1];
+            }
+        }
+        if (nlv < nlev) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int x = nx0 + u * NT + lane;
+                qn[u] = prog[x < ne1 ? x : ne1 - 1];
+            }
+        }
+        exec(q, x0 + lane, e1);
+        if (level_end) {
+            __syncthreads();
+            if (nlv >= nlev) break;
+        }
+        lv = nlv;
+        x0 = nx0;
+        e1 = ne1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = qn[u];
+    }
+}
+
 // Slot memory of a class: LDS, or a global scratch slice.
 template <class D>
 using CoopSlots = typename std::conditional<D::GLOBAL_SLOTS, QF_AS_GLOBAL double*, QF_AS_LDS double*>::type;
@@ -114,35 +163,25 @@ __device__ __forceinline__ void coop_contract(int pb, int pe, int qb, int qe, co
                 s[D::ZS] = 0.0;
             }
             __syncthreads();
-            for (int lv = 0; lv < D::NLV; ++lv) {
-                const int e1 = vlev[lv + 1];
-                // nodes of one level are independent: fetch kVU descriptors,
-                // read all their sources, then compute and store, so the
-                // memory latencies overlap instead of chaining node by node
-                for (int x0 = vlev[lv] + lane; x0 < e1; x0 += kVU * D::NT) {
-                    CoopVNode q[kVU];
+            // nodes of one level are independent: each lane reads the sources
+            // of its kVU nodes, then computes and stores, so the LDS latencies
+            // overlap instead of chaining node by node
+            run_levels<kVU, D::NT>(vrr, vlev, D::NLV, [&](const CoopVNode (&q)[kVU], int x0, int e1) {
+                double v[kVU];
 #pragma unroll
-                    for (int u = 0; u < kVU; ++u) {
-                        const int x = x0 + u * D::NT;
-                        q[u] = vrr[x < e1 ? x : e1 - 1];
-                    }
-                    double v[kVU];
-#pragma unroll
-                    for (int u = 0; u < kVU; ++u) {
-                        const SP c4 = cf + 4 * (q[u].meta & 15);
-                        const double n1 = double((q[u].meta >> 4) & 15), n2 = double(q[u].meta >> 8);
-                        v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] + n1 * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
-                               n2 * cf[24] * s[q[u].e];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kVU; ++u)
-                        if (x0 + u * D::NT < e1) {
-                            s[q[u].dst] = v[u];
-                            if (q[u].acc != 0xFFFF) s[q[u].acc] += v[u];
-                        }
+                for (int u = 0; u < kVU; ++u) {
+                    const SP c4 = cf + 4 * (q[u].meta & 15);
+                    const double n1 = double((q[u].meta >> 4) & 15), n2 = double(q[u].meta >> 8);
+                    v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] + n1 * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
+                           n2 * cf[24] * s[q[u].e];
                 }
-                __syncthreads();
-            }
+#pragma unroll
+                for (int u = 0; u < kVU; ++u)
+                    if (x0 + u * D::NT < e1) {
+                        s[q[u].dst] = v[u];
+                        if (q[u].acc != 0xFFFF) s[q[u].acc] += v[u];
+                    }
+            });
         }
     }
 }
@@ -166,24 +205,14 @@ __device__ __forceinline__ void coop_hrr(const PairInfo& bi, const PairInfo& ki,
     __syncthreads();
     const auto hrr = gptr(D::hrr());
     const auto hlev = cptr(D::hlev());
-    for (int lv = 0; lv < D::NLH; ++lv) {
-        const int e1 = hlev[lv + 1];
-        for (int x0 = hlev[lv] + lane; x0 < e1; x0 += kHU * D::NT) {
-            CoopHNode q[kHU];
+    run_levels<kHU, D::NT>(hrr, hlev, D::NLH, [&](const CoopHNode (&q)[kHU], int x0, int e1) {
+        double v[kHU];
 #pragma unroll
-            for (int u = 0; u < kHU; ++u) {
-                const int x = x0 + u * D::NT;
-                q[u] = hrr[x < e1 ? x : e1 - 1];
-            }
-            double v[kHU];
+        for (int u = 0; u < kHU; ++u) v[u] = s[q[u].s1] + cf[25 + q[u].sel] * s[q[u].s2];
 #pragma unroll
-            for (int u = 0; u < kHU; ++u) v[u] = s[q[u].s1] + cf[25 + q[u].sel] * s[q[u].s2];
-#pragma unroll
-            for (int u = 0; u < kHU; ++u)
-                if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
-        }
-        __syncthreads();
-    }
+        for (int u = 0; u < kHU; ++u)
+            if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+    });
 }
 
 // OS (the fused open-shell pass): after the six blocks of md.P the
