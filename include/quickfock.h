@@ -78,9 +78,14 @@ int qf_set_basis(qf_engine *e, int nshell, const int *shell_atom, const int *she
 int qf_set_cutoffs(qf_engine *e, double integral_cutoff, double prim_limit, double hyb);
 
 /* Precision: QF_PREC_FP64, or QF_PREC_MIXED where shell quartets with
- * Ycut_ij*Ycut_kl < split*max(Ycut_ij*Ycut_kl) run VRR/HRR in FP32
- * (Boys, bounds and Fock accumulation stay FP64). */
+ * Ycut_ij*Ycut_kl < split*max(Ycut_ij*Ycut_kl) run in FP32: exponents, Boys
+ * function (FP32 grid table), VRR and HRR (screening bounds, coordinate
+ * differences and the Fock accumulation stay FP64). */
 int qf_set_precision(qf_engine *e, int mode, double split);
+/* Which classes a mixed build splits (native tuning, no QUICK counterpart):
+ * those with La+Lb+Lc+Ld >= min_lsum that have per-lane kernels (default 0:
+ * all of them); the others run every quartet in FP64. */
+int qf_set_split_classes(qf_engine *e, int min_lsum);
 
 /* Schwarz bounds.  Pass NULL to have them computed on the GPU, or QUICK's
  * Ycutoff[nshell*nshell] and cutprim[nprim_total*nprim_total]. */

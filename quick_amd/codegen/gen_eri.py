@@ -762,7 +762,8 @@ def main():
             lines.append("    if (c == %d) return launch_schwarz_%s;" % (pc[(La, Lb)], name))
     lines += ["    return nullptr;", "}", "", "}  // namespace qf", ""]
     _write(os.path.join(OUT, "launchers.h"), "\n".join(lines))
-    from . import gen_grad
+    from . import gen_boys, gen_grad
+    _write(os.path.join(OUT, "boys_grid.h"), gen_boys.header())
     table["gradient"] = gen_grad.main(OUT, lambda name, text: _write(os.path.join(OUT, name), text))
     _write(os.path.join(OUT, "flop_table.json"), json.dumps(table, indent=1))
     for k, v in table.items():

@@ -253,6 +253,7 @@ struct qf_engine {
     double list_cutoff() const { return grad_cutoff > 0.0 ? std::min(cutoff, grad_cutoff) : cutoff; }
     int prec = QF_PREC_FP64;
     double split = 1e-4;
+    int split_lmin = 0;      // classes with La+Lb+Lc+Ld >= split_lmin are split in mixed builds
     double split_thr = 0.0;  // Ycut_ij*Ycut_kl below which quartets run in FP32
     int replicas = 8;        // copies of G the atomics are spread over
     int shuffle = 1;         // permute tasks inside a launch (decorrelates atomic targets)
@@ -712,13 +713,14 @@ int prepare(qf_engine* e)
                 // classes without per-lane kernels (La+Lb+Lc+Ld >= 6) run every
                 // quartet in FP64 on the wave-per-quartet kernel
                 const bool lane_class = qf::fock_launcher(cb, ck) != nullptr;
-                // Only classes with La+Lb+Lc+Ld >= 5 are split: their FP64 per-lane
-                // kernels spill and the FP32 form halves the registers; for the
-                // lighter classes the second launch costs more than FP32 saves
-                // ((Gly)10 6-31G(d,p): (dp|pp) -1.2 ms, (dp|ds) -0.36 ms split, the
-                // Lsum <= 4 classes +0.04..+0.43 ms each).
+                // Classes with La+Lb+Lc+Ld >= split_lmin are split (qf_set_split_classes).
+                // FP32 quartets compute their Boys function, exponents and
+                // recursions in FP32, which shortens the primitive loop (the
+                // FP64 Boys function is most of a light class's work) and halves
+                // the registers of the spilling d classes.
                 const int Lsum_s = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
-                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= 5;
+                const bool split_class =
+                    e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= e->split_lmin;
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
@@ -1204,6 +1206,14 @@ int qf_set_precision(qf_engine* e, int mode, double split)
     if (!e || (mode != QF_PREC_FP64 && mode != QF_PREC_MIXED) || !(split > 0)) return fail(QF_ERR_ARG, "precision");
     e->prec = mode;
     e->split = split;
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_split_classes(qf_engine* e, int min_lsum)
+{
+    if (!e || min_lsum < 0) return fail(QF_ERR_ARG, "min_lsum must be >= 0");
+    e->split_lmin = min_lsum;
     e->prepared = false;
     return QF_OK;
 }

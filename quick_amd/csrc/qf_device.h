@@ -7,6 +7,7 @@
 
 #include <cmath>
 
+#include "gen/boys_grid.h"
 #include "qf_types.h"
 
 namespace qf {
@@ -217,6 +218,61 @@ __host__ __device__ __forceinline__ void boys_quick(const double X, const int ma
     }
 }
 
+// e^-x and 1/sqrt(x) in FP32: the hardware exp2 and reciprocal square root
+// on the device, the C library on the host (CPU checks only).
+__host__ __device__ __forceinline__ float qf_expf_neg(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(-x);
+#else
+    return std::exp(-x);
+#endif
+}
+
+__host__ __device__ __forceinline__ float qf_rsqrtf(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / std::sqrt(x);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Boys function F_m(X), m = 0..L, in FP32, for the FP32 quartets of
+// split-precision builds (their integrals are below the split threshold,
+// so FP32 accuracy, ~1e-7 relative, is what they need; QUICK's FmT
+// branches and fits serve the FP64 quartets).  Inside the grid (X < 30):
+// F_L by a 5-term Taylor series about the nearest grid point of kBoysGrid
+// (dF_m/dX = -F_{m+1}), truncation below 5e-8 relative; above it the
+// asymptotic F_L = sqrt(pi/4X) prod (2m-1)/2X (e^-X < 1e-13).  Then the
+// downward recursion F_m = (2X F_{m+1} + e^-X) / (2m+1), which is stable.
+// ---------------------------------------------------------------------------
+template <int L, typename GT>
+__host__ __device__ __forceinline__ void boys_f32(const float X, float (&F)[L + 1], GT grid)
+{
+    static_assert(L + 4 < kBoysGridM, "Boys grid table too narrow for this class");
+    constexpr float kXmax = (kBoysGridN - 1) * kBoysGridH;
+    const float E = qf_expf_neg(X);
+    const int i = X < kXmax ? int(fmaf(X, kBoysGridInvH, 0.5f)) : kBoysGridN - 1;
+    const float d = float(i) * kBoysGridH - X;
+    const auto row = grid + i * kBoysGridM + L;
+    float f = row[4];
+    f = fmaf(f, d * 0.25f, row[3]);
+    f = fmaf(f, d * (1.0f / 3.0f), row[2]);
+    f = fmaf(f, d * 0.5f, row[1]);
+    f = fmaf(f, d, row[0]);
+    const float rx = qf_rsqrtf(X);
+    float a = 0.88622692545275801365f * rx;   // sqrt(pi/4X)
+    const float h = 0.5f * rx * rx;
+#pragma unroll
+    for (int m = 1; m <= L; ++m) a *= float(2 * m - 1) * h;
+    F[L] = X < kXmax ? f : a;
+    const float twox = X + X;
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) F[m] = fmaf(twox, F[m + 1], E) * (1.0f / float(2 * m + 1));
+}
+
 // cons factors of QUICK's Cartesian components (basis.f90:567-720): d
 // xy, xz, yz sqrt(3); f xxy, xyy, xxz, yyz, xzz, yzz sqrt(5), xyz sqrt(15)
 __host__ __device__ constexpr double kCons[4][10] = {
@@ -291,11 +347,13 @@ __device__ __forceinline__ void lds_add(QF_AS_LDS double* p, double v)
 // cost 23 VGPRs and was slower); the bra primitives (the same for the whole
 // wave) are the inner loop and come through scalar loads.
 // ---------------------------------------------------------------------------
-template <class C, typename R, bool BRA_UNIFORM, typename Tab>
+// grid: the FP32 Boys grid table (kBoysGrid or a copy in LDS), read by
+// the FP32 instantiation only.
+template <class C, typename R, bool BRA_UNIFORM, typename Tab, typename GT = const float*>
 __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, const PairInfo& bi, const PairBlock& kb,
                                          int j, const PairInfo& ki, double dn, double primlim, bool screen,
                                          int maxm, R (&acc)[C::NACC], Tab boys_tab, int pb, int pe, int qb,
-                                         int qe)
+                                         int qe, GT grid = kBoysGrid)
 {
     const auto bp = BRA_UNIFORM ? (const double*)nullptr : bb.prim;
     const auto bpc = cptr(bb.prim);           // wave-uniform bra data
@@ -328,31 +386,59 @@ __host__ __device__ __forceinline__ void contract(const PairBlock& bb, int i, co
             const double ozb = BP(PF_OOZ2, ip);
             const double Pb[3] = {BP(PF_PX, ip), BP(PF_PY, ip), BP(PF_PZ, ip)};
             const double Kb = BP(PF_KC, ip);
-            const double rs = qf_rsqrt(zb + zk);       // 1/sqrt(zeta+eta)
-            const double inv = rs * rs;
             const double PQ[3] = {Pb[0] - Qk[0], Pb[1] - Qk[1], Pb[2] - Qk[2]};
-            const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
-            const double T = rpq * (zb * zk * inv);
-            double F[C::L + 1];
-            boys_quick<C::L>(T, maxm, F, boys_tab);
-            const double pref = Kb * Kk * rs;
             R Fr[C::L + 1];
-#pragma unroll
-            for (int m = 0; m <= C::L; ++m) Fr[m] = R(F[m] * pref);
             VrrPar<R> p;
-            const double wp = zk * inv, wq = zb * inv;
+            if constexpr (sizeof(R) == 4) {
+                // FP32 quartet: exponents, Boys function and VRR parameters in
+                // FP32; only the coordinate differences are formed in FP64
+                const float zbf = float(zb), zkf = float(zk);
+                const float rs = qf_rsqrtf(zbf + zkf);
+                const float inv = rs * rs;
+                const float PQf[3] = {float(PQ[0]), float(PQ[1]), float(PQ[2])};
+                const float rpq = fmaf(PQf[0], PQf[0], fmaf(PQf[1], PQf[1], PQf[2] * PQf[2]));
+                float F[C::L + 1];
+                boys_f32<C::L>(rpq * (zbf * zkf * inv), F, grid);
+                const float pref = float(Kb * Kk) * rs;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                p.PA[d] = R(Pb[d] - bi.A[d]);
-                p.WP[d] = R(-wp * PQ[d]);
-                p.QC[d] = R(QC[d]);
-                p.WQ[d] = R(wq * PQ[d]);
+                for (int m = 0; m <= C::L; ++m) Fr[m] = F[m] * pref;
+                const float wp = zkf * inv, wq = zbf * inv;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    p.PA[d] = float(Pb[d] - bi.A[d]);
+                    p.WP[d] = -wp * PQf[d];
+                    p.QC[d] = float(QC[d]);
+                    p.WQ[d] = wq * PQf[d];
+                }
+                p.oo2z = float(ozb);
+                p.oo2n = float(ozk);
+                p.oo2zn = 0.5f * inv;
+                p.rz = wp;
+                p.rn = wq;
+            } else {
+                const double rs = qf_rsqrt(zb + zk);       // 1/sqrt(zeta+eta)
+                const double inv = rs * rs;
+                const double rpq = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+                const double T = rpq * (zb * zk * inv);
+                double F[C::L + 1];
+                boys_quick<C::L>(T, maxm, F, boys_tab);
+                const double pref = Kb * Kk * rs;
+#pragma unroll
+                for (int m = 0; m <= C::L; ++m) Fr[m] = R(F[m] * pref);
+                const double wp = zk * inv, wq = zb * inv;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    p.PA[d] = R(Pb[d] - bi.A[d]);
+                    p.WP[d] = R(-wp * PQ[d]);
+                    p.QC[d] = R(QC[d]);
+                    p.WQ[d] = R(wq * PQ[d]);
+                }
+                p.oo2z = R(ozb);
+                p.oo2n = R(ozk);
+                p.oo2zn = R(0.5 * inv);
+                p.rz = R(wp);
+                p.rn = R(wq);
             }
-            p.oo2z = R(ozb);
-            p.oo2n = R(ozk);
-            p.oo2zn = R(0.5 * inv);
-            p.rz = R(wp);
-            p.rn = R(wq);
             C::template vrr<R>(Fr, p, acc);
         }
     }
@@ -386,6 +472,7 @@ struct TileRef {
     QF_AS_LDS double* jab;   // nbra x na x nb Coulomb terms of the bras
     int ncol, chunk_kb;
     const QF_AS_LDS double* boys;   // Boys fit table (kBoysFit) in LDS
+    const QF_AS_LDS float* grid;    // FP32 Boys grid table (kBoysGrid) in LDS (FP32 kernels)
 };
 
 // OS = true (the fused open-shell pass): every integral is digested a second
@@ -473,7 +560,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         for (int k = 0; k < C::NACC; ++k) acc[k] = R(0);
         if (valid)
             contract<C, R, true>(bb, i, bi, kb, j, ki, dn, md.primlim, true, bi.qfs + ki.qfs, acc, tr.boys, t.pb, t.pe,
-                                 t.qb, t.qe);
+                                 t.qb, t.qe, tr.grid);
         R I[C::NI];
         {
             const R AB[3] = {R(bi.AB[0]), R(bi.AB[1]), R(bi.AB[2])};
@@ -686,7 +773,13 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
                       const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols, int k2)
 {
     constexpr int na = ncart(C::La), nb = ncart(C::Lb);
+    constexpr bool kF32 = sizeof(R) == 4;
     extern __shared__ double qf_lds[];
+    // FP32 kernels: the FP32 Boys grid in LDS (6 KB; global-memory gathers
+    // per primitive stalled the loop)
+    __shared__ float qf_boys_grid[kF32 ? kBoysGridN * kBoysGridM : 1];
+    if constexpr (kF32)
+        for (int x = threadIdx.x; x < kBoysGridN * kBoysGridM; x += blockDim.x) qf_boys_grid[x] = kBoysGrid[x];
     QF_BOYS_LDS_DECL;
     QF_BOYS_LDS_FILL();
     QF_AS_LDS double* lds = (QF_AS_LDS double*)qf_lds;
@@ -708,7 +801,8 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
         const size_t roff = (size_t)(blockIdx.x % md.replicas) * n * n;
         const auto G = gptr_rw(md.G + roff);
         const auto G2 = gptr_rw(OS ? md.G2 + roff : md.G);
-        const TileRef tr{lds, two ? lds + ntile : nullptr, lds + nk, ncol, g.chunk_kb, QF_BOYS_LDS_PTR};
+        const TileRef tr{lds, two ? lds + ntile : nullptr, lds + nk, ncol, g.chunk_kb, QF_BOYS_LDS_PTR,
+                         (const QF_AS_LDS float*)qf_boys_grid};
         // the wave index through readfirstlane: the compiler cannot prove
         // threadIdx.x / 64 wave-uniform, and would otherwise keep the task,
         // its loop bounds and the bra primitive data in vector registers
