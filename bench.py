@@ -17,7 +17,11 @@ the other workloads of the metric measured in the same run:
 * ``batched``: 256 uneven molecules, plain FP64 per-molecule builds against
   one batched HIP-graph build in mixed precision, both autotuned;
 * ``serving``: Poisson SCF requests through the continuous-batching server,
-  requests/s and p50/p99 latency at the stated rate.
+  requests/s and p50/p99 latency at the stated rate;
+* ``open_shell``: (H2O)32+ UHF, open-shell builds against the closed-shell
+  build of the same basis.
+The headline object also carries ``gradient``: the two-electron nuclear
+gradient of (H2O)32 at the converged density.
 The CPU baseline is the oracle (the C restatement of QUICK's CPU get2e) on
 the host cores this job has; the same leg states the GPU results' error in
 Eh against that FP64 oracle (Fock matrices and SCF energies).
@@ -129,7 +133,8 @@ def main():
     ns = None
     if not a.no_north_star:
         c3, ctx3 = measure_gly10(a, dev)
-        ns = {"split_precision": c3, "batched": measure_batch(a, dev), "serving": measure_serve(a, dev)}
+        ns = {"split_precision": c3, "batched": measure_batch(a, dev), "serving": measure_serve(a, dev),
+              "open_shell": measure_uhf(a, dev)}
         ctx["gly10"] = ctx3
     cpu = None if a.no_cpu_baseline else cpu_baseline_leg(a, ctx)
     if cpu is not None:
@@ -267,6 +272,18 @@ def measure_h2o32(a, dev):
                  for (nm, f32, nt), m, fl, q, pq in zip(names, avg_ms, flops, nquart, nprim)]
     log("[bench] per-class: " + json.dumps(per_class))
 
+    # ---- two-electron nuclear gradient at the same density -------------------
+    gd = torch.zeros((mol.natom, 3), dtype=torch.float64, device=dev)
+    eng.eri_grad_device(P.data_ptr(), gd.data_ptr(), sh)          # warm-up (allocates its scratch)
+    ms_grad, _ = timed(lambda: eng.eri_grad_device(P.data_ptr(), gd.data_ptr(), sh), 3)
+    gfl, gq, gpq = eng.grad_launch_counts()
+    gradient = {"workload": "two-electron nuclear gradient (QUICK's get_cshell_eri_grad) at the converged density",
+                "ms": round(ms_grad, 3), "gflop": round(float(np.sum(gfl)) / 1e9, 3),
+                "tflops": round(float(np.sum(gfl)) / (ms_grad * 1e-3) / 1e12, 3),
+                "quartets": int(np.sum(gq)), "prim_quartets": int(np.sum(gpq)),
+                "over_fock_build": None}
+    log(f"[bench] gradient {ms_grad:.2f} ms")
+
     # HBM bytes per launch of that kernel, measured by rocprofv3 FETCH_SIZE /
     # WRITE_SIZE passes over this workload (scaled by a copy of known size);
     # reported only while the summary names the same kernel
@@ -303,8 +320,10 @@ def measure_h2o32(a, dev):
         "wall_s_timed": round(wall, 3),
         "scf": scf,
         "per_class": per_class,
+        "gradient": gradient,
         "autotuned": {k: list(v[0]) for k, v in sorted(tuned.items())} if tuned else None,
     }
+    gradient["over_fock_build"] = round(ms_grad / ms_per_step, 2)
     ctx = {"mol": mol, "P": P_host, "G": G_graph, "E_scf": scf["energy_eh"] if scf else None}
     return out, ctx
 
@@ -651,6 +670,20 @@ def run_gly10(a, dev):
 
 
 def run_uhf(a, dev):
+    """The open-shell workload alone, as the headline line."""
+    r = measure_uhf(a, dev)
+    out = {
+        "metric": METRIC, "value": r["builds_per_s"], "unit": "open-shell Fock builds/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {k: r[k] for k in ("workload", "natom", "nbasis", "integral_cutoff", "prim_limit")},
+        "closed_shell_ms_same_basis": r["closed_shell_ms"], "open_over_closed": r["open_over_closed"],
+        "uhf_energy_eh": r["uhf_energy_eh"], "uhf_iterations": r["uhf_iterations"], "uhf_wall_s": r["uhf_wall_s"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def measure_uhf(a, dev):
     """(H2O)32+ doublet UHF/6-31G(d): the open-shell build (one pass that
     digests every integral with Pa+Pb and, for the exchange terms, with
     Pa-Pb) on the converged UHF densities, against the closed-shell build
@@ -678,17 +711,14 @@ def run_uhf(a, dev):
     t_os = _timed(lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh),
                   a.steps, a.warmup)
     t_cs = _timed(lambda: e.fock2e_device(Pt.data_ptr(), F.data_ptr(), sh), a.steps, a.warmup)
-    out = {
-        "metric": METRIC, "value": round(1.0 / t_os, 4), "unit": "open-shell Fock builds/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_os * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"(H2O){a.nwater}+ doublet UHF/6-31G(d) open-shell Fock build (Ga, Gb), "
-                               "converged UHF densities", "natom": mol.natom, "nbasis": bs.nbasis,
-                   "integral_cutoff": a.cutoff, "prim_limit": 0.1 * a.cutoff},
-        "closed_shell_ms_same_basis": round(t_cs * 1e3, 4), "open_over_closed": round(t_os / t_cs, 3),
-        "uhf_energy_eh": r.energy, "uhf_iterations": r.iterations, "uhf_wall_s": round(r.wall, 3),
-    }
-    print(json.dumps(out), flush=True)
+    out = {"workload": f"(H2O){a.nwater}+ doublet UHF/6-31G(d) open-shell Fock build (Ga, Gb), "
+                       "converged UHF densities", "natom": mol.natom, "nbasis": bs.nbasis,
+           "integral_cutoff": a.cutoff, "prim_limit": 0.1 * a.cutoff,
+           "builds_per_s": round(1.0 / t_os, 4), "ms": round(t_os * 1e3, 4),
+           "closed_shell_ms": round(t_cs * 1e3, 4), "open_over_closed": round(t_os / t_cs, 3),
+           "uhf_energy_eh": r.energy, "uhf_iterations": r.iterations, "uhf_wall_s": round(r.wall, 3)}
+    e.close()
+    return out
 
 
 def run_quick_tests(a, dev):

@@ -284,13 +284,6 @@ __host__ __device__ constexpr double kCons[4][10] = {
 
 constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
-}
-
 // v moved between lanes by the DPP control CTRL (rows in ROW_MASK only;
 // lanes without a source, or in other rows, get 0).
 template <int CTRL, int ROW_MASK>

@@ -101,29 +101,19 @@ __device__ __forceinline__ void grad_contract(int pb, int pe, int qb, int qe, co
                 s[D::ZS] = 0.0;
             }
             __syncthreads();
-            for (int lv = 0; lv < D::NLV; ++lv) {
-                const int e1 = vlev[lv + 1];
-                for (int x0 = vlev[lv] + lane; x0 < e1; x0 += kVU * D::NT) {
-                    CoopVNode q[kVU];
+            run_levels<kVU, D::NT>(vrr, vlev, D::NLV, [&](const CoopVNode (&q)[kVU], int x0, int e1) {
+                double v[kVU];
 #pragma unroll
-                    for (int u = 0; u < kVU; ++u) {
-                        const int x = x0 + u * D::NT;
-                        q[u] = vrr[x < e1 ? x : e1 - 1];
-                    }
-                    double v[kVU];
-#pragma unroll
-                    for (int u = 0; u < kVU; ++u) {
-                        const SP c4 = cf + 4 * (q[u].meta & 15);
-                        const double n1 = double((q[u].meta >> 4) & 15), n2 = double(q[u].meta >> 8);
-                        v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] + n1 * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
-                               n2 * cf[24] * s[q[u].e];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kVU; ++u)
-                        if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+                for (int u = 0; u < kVU; ++u) {
+                    const SP c4 = cf + 4 * (q[u].meta & 15);
+                    const double n1 = double((q[u].meta >> 4) & 15), n2 = double(q[u].meta >> 8);
+                    v[u] = c4[0] * s[q[u].a] + c4[1] * s[q[u].b] + n1 * c4[2] * (s[q[u].c] - c4[3] * s[q[u].d]) +
+                           n2 * cf[24] * s[q[u].e];
                 }
-                __syncthreads();
-            }
+#pragma unroll
+                for (int u = 0; u < kVU; ++u)
+                    if (x0 + u * D::NT < e1) s[q[u].dst] = v[u];
+            });
             // the four weighted sets: every entry has its own accumulator
             for (int x = lane; x < D::NX; x += D::NT) {
                 const GradAcc g = acc[x];
@@ -196,7 +186,7 @@ __device__ __forceinline__ void grad_quartet(const Task& t, const PairBlock& bb,
     (void)nap;
     (void)nam;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) g[k] = wave_sum(g[k]);
+    for (int k = 0; k < 9; ++k) g[k] = wave_sum_all_active(g[k]);
     __syncthreads();   // the next quartet reuses the slots
 }
 
@@ -397,7 +387,8 @@ __global__ __launch_bounds__(WAVE) void grad_lane_kernel(const Task* __restrict_
         double Dcd[nc * nd];
 #pragma unroll
         for (int x = 0; x < nc * nd; ++x) Dcd[x] = P[(ki.fa + x / nd) * n + ki.fb + x % nd];
-        for (int kk = t.kb; kk < t.ke; ++kk) {
+        for (int kk0 = t.kb; kk0 < t.ke; ++kk0) {
+            const int kk = wave_uniform(kk0);
             const int i = bperm[kk];
             const PairInfo bi = cptr(bb.info)[i];
             atA = sat[bi.sa];
@@ -443,7 +434,7 @@ __global__ __launch_bounds__(WAVE) void grad_lane_kernel(const Task* __restrict_
                 gA[x] += g[x];
                 gC[x] += g[6 + x];
                 gD[x] -= g[x] + g[3 + x] + g[6 + x];
-                const double b = wave_sum(g[3 + x]);
+                const double b = wave_sum_all_active(g[3 + x]);
                 if (lane == 0) {
                     if (tile)
                         lds_add(gt + 3 * atB + x, b);
@@ -454,7 +445,7 @@ __global__ __launch_bounds__(WAVE) void grad_lane_kernel(const Task* __restrict_
         }
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            const double a = wave_sum(gA[x]);
+            const double a = wave_sum_all_active(gA[x]);
             if (tile) {
                 if (lane == 0) lds_add(gt + 3 * atA + x, a);
                 if (in_range) {
