@@ -519,6 +519,15 @@ def coop_source(name, La, Lb, Lc, Ld):
     return cname, "\n".join(src) + "\n", t["total"]
 
 
+# per-class occupancy targets that differ from the size rule, measured on
+# (H2O)32 6-31G(d) (FP64 ms per build before -> after): (ds|ps) 2 waves
+# 0.87 -> 0.62 (no spills), (ds|ds) 1 wave 0.45 -> 0.23 (no spills),
+# (pp|ss) 4 waves 0.35 -> 0.32, (ss|ss) 5 waves 1.31 -> 1.23.  Tried and
+# slower: (ps|ps), (dp|ss), (dp|ds) at 2; (pp|ps) at 3; (ds|pp) at 2;
+# (ss|ss) at 6, (ps|ss) at 5, (ds|ss) at 4 (they spill).
+MIN_WAVES_OVERRIDE = {(2, 0, 1, 0): 2, (2, 0, 2, 0): 1, (1, 1, 0, 0): 4, (0, 0, 0, 0): 5}
+
+
 def min_waves(nacc, ni):
     """Occupancy target per class (caps VGPRs: 4 waves -> 128, 3 -> 168, 2 -> 256).
     (ss|ss) fits 128 VGPRs without spilling and gets 4 waves; every other
@@ -569,7 +578,7 @@ def gen_class(La, Lb, Lc, Ld):
     src.append("  static constexpr int NACC = %d, NI = %d;" % (nacc, ni))
     src.append("  static constexpr int VRR_FLOP = %d, HRR_FLOP = %d;" % (vflops, hflops))
     src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets"
-               % min_waves(nacc, ni))
+               % MIN_WAVES_OVERRIDE.get((La, Lb, Lc, Ld), min_waves(nacc, ni)))
     src.append("  template <typename R>")
     src.append("  __host__ __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
     src += ["    " + l for l in vlines]
