@@ -704,10 +704,11 @@ def measure_uhf(a, dev):
     Pa, Pb = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in r.density)
     Pt = Pa + Pb
     sh = torch.cuda.current_stream(dev).cuda_stream
-    if a.autotune:
-        tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32+", builds=2,
-                     make_density=lambda i: Pt.data_ptr())
     Fa, Fb, F = torch.zeros_like(Pa), torch.zeros_like(Pa), torch.zeros_like(Pa)
+    if a.autotune:
+        # tuned on the open-shell build itself (its kernels carry more state)
+        tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32+", builds=2,
+                     build=lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh))
     t_os = _timed(lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh),
                   a.steps, a.warmup)
     t_cs = _timed(lambda: e.fock2e_device(Pt.data_ptr(), F.data_ptr(), sh), a.steps, a.warmup)

@@ -53,12 +53,13 @@ def _class_index(name):
 
 
 def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None, densities=None,
-             pieces=PIECES):
+             pieces=PIECES, build=None):
     """Choose a launch shape per class, then a primitive piece size per class.
     ``engines``: one FockEngine or a list (tuned as one FockBatch).  Single
     engine: ``make_density(0)`` returns the device pointer (int) of the
-    density to build with.  Batch: ``densities`` holds one host density per
-    engine.  Returns {class name: (candidate, ms)} (candidate = shape +
+    density to build with, or ``build()`` runs one build of the kind to tune
+    (e.g. the open-shell build).  Batch: ``densities`` holds one host density
+    per engine.  Returns {class name: (candidate, ms)} (candidate = shape +
     (piece,)) and leaves the best settings applied."""
     from .engine import FockBatch
     single = not isinstance(engines, (list, tuple))
@@ -67,6 +68,8 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
     def timed():
         if single:
             e = engs[0]
+            if build is not None:
+                return _class_times(build, e.launches, e.read_timing, e.set_timing, builds)
             P = make_density(0)
             return _class_times(lambda: e.fock2e_device(P, 0, stream), e.launches, e.read_timing, e.set_timing,
                                 builds)

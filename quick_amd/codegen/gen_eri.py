@@ -526,6 +526,9 @@ def coop_source(name, La, Lb, Lc, Ld):
 # slower: (ps|ps), (dp|ss), (dp|ds) at 2; (pp|ps) at 3; (ds|pp) at 2;
 # (ss|ss) at 6, (ps|ss) at 5, (ds|ss) at 4 (they spill).
 MIN_WAVES_OVERRIDE = {(2, 0, 1, 0): 2, (2, 0, 2, 0): 1, (1, 1, 0, 0): 4, (0, 0, 0, 0): 5}
+# the open-shell kernels carry a second set of exchange accumulators and
+# density blocks: one wave fewer than the closed-shell kernel by default
+MIN_WAVES_OS_OVERRIDE = {(1, 0, 0, 0): 3}
 
 
 def min_waves(nacc, ni):
@@ -577,8 +580,10 @@ def gen_class(La, Lb, Lc, Ld):
     src.append("  static constexpr int La = %d, Lb = %d, Lc = %d, Ld = %d, L = %d;" % (La, Lb, Lc, Ld, L))
     src.append("  static constexpr int NACC = %d, NI = %d;" % (nacc, ni))
     src.append("  static constexpr int VRR_FLOP = %d, HRR_FLOP = %d;" % (vflops, hflops))
-    src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets"
-               % MIN_WAVES_OVERRIDE.get((La, Lb, Lc, Ld), min_waves(nacc, ni)))
+    mw = MIN_WAVES_OVERRIDE.get((La, Lb, Lc, Ld), min_waves(nacc, ni))
+    src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets" % mw)
+    src.append("  static constexpr int MIN_WAVES_OS = %d;   // the same for the open-shell kernels"
+               % MIN_WAVES_OS_OVERRIDE.get((La, Lb, Lc, Ld), max(1, mw - 1)))
     src.append("  template <typename R>")
     src.append("  __host__ __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
     src += ["    " + l for l in vlines]

@@ -732,7 +732,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 // The grid may be smaller than the task count: each workgroup then strides
 // over the tasks (a persistent launch).
 template <class C, typename R, bool OS = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
+__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, WAVE), amdgpu_waves_per_eu(OS ? C::MIN_WAVES_OS : C::MIN_WAVES)))
 void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __restrict__ blocks,
                  const MolDesc* __restrict__ mols)
 {
@@ -761,7 +761,8 @@ void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __
 // second exchange terms go to a second LDS tile (k2 = 1) or straight to
 // G2 (k2 = 0, where two tiles do not fit).
 template <class C, typename R, bool OS = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, 4 * WAVE), amdgpu_waves_per_eu(C::MIN_WAVES)))
+__global__ __attribute__((amdgpu_flat_work_group_size(WAVE, 4 * WAVE),
+                          amdgpu_waves_per_eu(OS ? C::MIN_WAVES_OS : C::MIN_WAVES)))
 void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const Task* __restrict__ tasks,
                       const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols, int k2)
 {
