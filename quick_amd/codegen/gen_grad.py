@@ -257,8 +257,10 @@ def grad_tables(La, Lb, Lc, Ld):
 
 # Classes with La+Lb+Lc+Ld up to this also get a per-lane gradient kernel
 # (one quartet per lane, recursions in registers); the wave-per-quartet
-# kernel serves the rest.
-LANE_GRAD_LSUM_MAX = 3
+# kernel serves the rest.  At 4 and 5 the per-lane kernels spill 1.3-10 KB
+# per lane and are still faster than the wave-per-quartet ones ((H2O)32
+# 6-31G(d) gradient: 192 ms with 3, 134 ms with 4, 123 ms with 5).
+LANE_GRAD_LSUM_MAX = 5
 
 
 def has_lane_grad(La, Lb, Lc, Ld):
