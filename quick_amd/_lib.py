@@ -103,6 +103,7 @@ def lib():
         "qf_set_persistent": (ctypes.c_int, [vp, ctypes.c_int]),
         "qf_set_prim_split": (ctypes.c_int, [vp, ctypes.c_int]),
         "qf_set_class_piece": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "qf_set_class_os": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "qf_read_timing": (ctypes.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():

@@ -53,14 +53,17 @@ def _class_index(name):
 
 
 def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None, densities=None,
-             pieces=PIECES, build=None):
+             pieces=PIECES, build=None, os_modes=None):
     """Choose a launch shape per class, then a primitive piece size per class.
     ``engines``: one FockEngine or a list (tuned as one FockBatch).  Single
     engine: ``make_density(0)`` returns the device pointer (int) of the
     density to build with, or ``build()`` runs one build of the kind to tune
     (e.g. the open-shell build).  Batch: ``densities`` holds one host density
     per engine.  Returns {class name: (candidate, ms)} (candidate = shape +
-    (piece,)) and leaves the best settings applied."""
+    (piece,)) and leaves the best settings applied.  ``os_modes`` (e.g. (0, 1)
+    with an open-shell ``build``): a third pass choosing each class's
+    open-shell form (fused or two closed-shell passes), appended to the
+    candidate."""
     from .engine import FockBatch
     single = not isinstance(engines, (list, tuple))
     engs = [engines] if single else list(engines)
@@ -121,6 +124,26 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
         cand, _ = chosen[name]
         chosen[name] = (tuple(cand) + (best,), ptotal(best) if pieces else chosen[name][1])
     autotune.last_piece_times = ptimes
+    if os_modes:
+        otimes = {}
+        for om in os_modes:
+            for e in engs:
+                for cb in range(NPC):
+                    for ck in range(cb + 1):
+                        e.set_class_os(cb, ck, om)
+            otimes[om] = timed()
+        for name in names:
+            cb, ck = _class_index(name)
+
+            def ototal(om):
+                keys = [k for k in otimes[om] if k[0] == name]
+                return sum(otimes[om][k] for k in keys) if keys else float("inf")
+            best = min(os_modes, key=ototal)
+            for e in engs:
+                e.set_class_os(cb, ck, best)
+            cand, _ = chosen[name]
+            chosen[name] = (tuple(cand) + (best,), ototal(best))
+        autotune.last_os_times = otimes
     return chosen
 
 
@@ -154,7 +177,7 @@ def table_key(engines, batched=False, device_name=None, tag=None):
 
 
 def load_table(key, path=TABLE):
-    """{class name: (digestion, waves, tasks/group, bras/task, piece)} or None."""
+    """{class name: (digestion, waves, tasks/group, bras/task, piece[, open-shell form])} or None."""
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -182,6 +205,8 @@ def apply_table(engines, entry):
         for e in engs:
             e.set_class_tuning(cb, ck, *cand[:4])
             e.set_class_piece(cb, ck, cand[4])
+            if len(cand) > 5:
+                e.set_class_os(cb, ck, cand[5])
 
 
 def tune_or_load(engines, stream, batched=False, retune=False, path=TABLE, save_path=None, tag=None, **kw):

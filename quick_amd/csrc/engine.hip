@@ -266,6 +266,7 @@ struct qf_engine {
     struct ClassTune {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
         int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
+        int os = -1;                                    // open-shell form: -1 default (fused), 0 fused, 1 two passes
     };
     ClassTune ctune[kNPC][kNPC];
     // pair blocks
@@ -278,6 +279,7 @@ struct qf_engine {
         int tile, goffset, gcount, waves;
         size_t lds;
         size_t lds_os;   // open-shell pass with a second exchange tile (0: it does not fit)
+        int os_split;    // open-shell builds run this class as two closed-shell passes
     };
     std::vector<Launch> launches;
     bool prepared = false;
@@ -298,6 +300,7 @@ struct qf_engine {
     DevBuf<double> d_Gt, d_Gs, d_cm, d_P;
     DevBuf<double> d_Pb, d_Pt, d_Ps, d_Fa, d_Fb;   // open-shell builds (allocated on first use)
     DevBuf<double> d_Gt2;                           // their second exchange accumulators (replicas x n^2)
+    DevBuf<double> d_zero;                          // nbasis^2 zeros: Coulomb density of an exchange-only pass
     DevBuf<int> d_f0, d_f1;
     DevBuf<unsigned long long> d_counts;
     DevBuf<double> d_cscratch;   // global slots of the wave-per-quartet classes that need them
@@ -875,6 +878,7 @@ int prepare(qf_engine* e)
                                          : 1;
                     L.lds = lds;
                     L.lds_os = lds_os <= kMaxTileLds ? lds_os : 0;
+                    L.os_split = ct.os == 1;
                     e->launches.push_back(L);
                 }
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
@@ -993,17 +997,21 @@ struct GBufs {
     double* Gt2 = nullptr;   // open-shell pass: accumulators of the second exchange density
 };
 
-// dP2 != nullptr: the fused open-shell pass.  Every integral is digested
-// with dP (Coulomb and exchange, into Gt) and with dP2 (exchange only, into
-// gb.Gt2); then dF += A + B and dF2 += A - B (k_finalize_os).
+// dP2 != nullptr: the open-shell pass.  Every integral is digested with dP
+// (Coulomb and exchange, into Gt) and with dP2 (exchange only, into gb.Gt2);
+// then dF += A + B and dF2 += A - B (k_finalize_os).  A class with os_split
+// runs as two closed-shell launches instead of the fused one: dP into Gt,
+// then dP2 with a zero Coulomb density (e->d_zero) into Gt2 (descriptor slot
+// 1).
 int build_pass(qf_engine* e, const double* dP, const double* dcm, double* dF, int slot, hipStream_t s,
                const GBufs& gb, const double* dP2 = nullptr, double* dF2 = nullptr)
 {
     const int os = dP2 != nullptr;
-    if (os && (!gb.Gt2 || !dF || !dF2)) return fail(QF_ERR_ARG, "open-shell pass without its buffers");
+    if (os && (!gb.Gt2 || !dF || !dF2 || !e->d_zero.p)) return fail(QF_ERR_ARG, "open-shell pass without its buffers");
     const int n = e->nbasis, ns = e->nshell;
     qf::MolDesc md{};
     md.P = dP;
+    md.PJ = dP;
     md.G = gb.Gt;
     md.cutmat = dcm;
     md.nbasis = n;
@@ -1018,6 +1026,18 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, double* dF, in
     e->mol[slot] = md;
     qf::MolDesc* dmol = e->d_mol.p + slot;
     QF_HIP(hipMemcpyAsync(dmol, &e->mol[slot], sizeof(md), hipMemcpyHostToDevice, s));
+    qf::MolDesc* dmol_x = nullptr;   // exchange-only pass of the two-pass open-shell classes
+    if (os) {
+        qf::MolDesc mx = md;
+        mx.P = dP2;
+        mx.PJ = e->d_zero.p;
+        mx.G = gb.Gt2;
+        mx.P2 = nullptr;
+        mx.G2 = nullptr;
+        e->mol[1] = mx;
+        dmol_x = e->d_mol.p + 1;
+        QF_HIP(hipMemcpyAsync(dmol_x, &e->mol[1], sizeof(mx), hipMemcpyHostToDevice, s));
+    }
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, gb.Gt, (size_t)n * n * gb.replicas);
     if (os) hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, gb.Gt2, (size_t)n * n * gb.replicas);
     for (size_t li = 0; li < e->launches.size(); ++li) {
@@ -1035,23 +1055,35 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, double* dF, in
             e->ev_launch.push_back((int)li);
             QF_HIP(hipEventRecord(ea, s));
         }
-        if (L.tile == 2) {
-            qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
-            if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
-            fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, os, e->persist, e->d_cscratch.p, s);
-        } else if (L.tile) {
-            qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
-            if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
-            // open shell: a second exchange tile where two fit, else its terms go to Gt2
-            const int tos = !os ? 0 : L.lds_os ? 1 : 2;
-            ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, dmol, L.fp32, tos, L.waves,
-               tos == 1 ? L.lds_os : L.lds, e->persist, s);
+        // one class launch with descriptor m; los: its open-shell form
+        auto launch = [&](qf::MolDesc* m, int los) -> int {
+            if (L.tile == 2) {
+                qf::FockCoopLaunch fc = qf::fock_coop_launcher(L.cb, L.ck);
+                if (!fc) return fail(QF_ERR_ARG, "no wave-per-quartet kernel for class");
+                fc(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, m, los, e->persist, e->d_cscratch.p, s);
+            } else if (L.tile) {
+                qf::FockTileLaunch ft = qf::fock_tile_launcher(L.cb, L.ck);
+                if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
+                // open shell: a second exchange tile where two fit, else its terms go to Gt2
+                const int tos = !los ? 0 : L.lds_os ? 1 : 2;
+                ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, m, L.fp32, tos, L.waves,
+                   tos == 1 ? L.lds_os : L.lds, e->persist, s);
+            } else {
+                qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
+                if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
+                if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, m, L.fp32, los, e->persist, s))
+                    return fail(QF_ERR_ARG, kDirectFp32Refused);
+            }
+            return QF_OK;
+        };
+        int rc = QF_OK;
+        if (os && L.os_split) {
+            rc = launch(dmol, 0);
+            if (rc == QF_OK) rc = launch(dmol_x, 0);
         } else {
-            qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
-            if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
-            if (f(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, L.fp32, os, e->persist, s))
-                return fail(QF_ERR_ARG, kDirectFp32Refused);
+            rc = launch(dmol, os);
         }
+        if (rc) return rc;
         if (gb.timing) QF_HIP(hipEventRecord(eb, s));
     }
     if (os)
@@ -1106,6 +1138,10 @@ int build_os(qf_engine* e, const double* dPa, const double* dPb, double* dFa, do
         dcm = e->d_cm.p;
     }
     QF_HIP(e->d_Gt2.resize((size_t)n2 * e->replicas));
+    if (e->d_zero.n < (size_t)n2) {
+        QF_HIP(e->d_zero.resize(n2));
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, e->d_zero.p, (size_t)n2);
+    }
     GBufs gb{e->d_Gt.p, e->replicas, e->d_Gs.p, e->timing};
     gb.Gt2 = e->d_Gt2.p;
     const int rc = build_pass(e, e->d_Pt.p, dcm, dFa, 0, s, gb, e->d_Ps.p, dFb);
@@ -1312,6 +1348,7 @@ int grad_pass(qf_engine* e, const double* dP, const double* dcm, double* dgrad, 
         QF_HIP(e->d_gscratch.resize(gdoubles * qf::COOP_GLOBAL_SLICES));
     qf::MolDesc md{};
     md.P = dP;
+    md.PJ = dP;
     md.G = nullptr;
     md.cutmat = dcm;
     md.nbasis = e->nbasis;
@@ -1629,6 +1666,16 @@ int qf_set_class_tuning(qf_engine* e, int cb, int ck, int tile, int waves, int t
     return QF_OK;
 }
 
+int qf_set_class_os(qf_engine* e, int cb, int ck, int mode)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (mode < -1 || mode > 1) return fail(QF_ERR_ARG, "mode must be -1, 0 (fused) or 1 (two passes)");
+    e->ctune[cb][ck].os = mode;
+    e->prepared = false;
+    return QF_OK;
+}
+
 int qf_set_class_piece(qf_engine* e, int cb, int ck, int piece)
 {
     if (!e) return fail(QF_ERR_ARG, "NULL");
@@ -1886,6 +1933,7 @@ int qf_batch_create_ext(qf_engine* const* engines, int count, int replicas, cons
         if (d_P && (!d_P[i] || !d_F[i])) return bail(fail(QF_ERR_ARG, "NULL density/Fock pointer"));
         qf::MolDesc m{};
         m.P = d_P ? d_P[i] : b->d_P.p + poff[i];
+        m.PJ = m.P;
         m.G = b->d_Gt.p + gtoff[i];
         m.cutmat = b->d_cm.p + cmoff[i];
         m.nbasis = e->nbasis;

@@ -245,8 +245,9 @@ __device__ __forceinline__ void coop_quartet(const Task& t, const PairBlock& bb,
     SP Dad = Dac + na * nc;
     SP Dbc = Dad + na * nd;
     SP Dbd = Dbc + nb * nc;
-    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = P[(bi.fa + x / nb) * n + bi.fb + x % nb];
-    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = P[(ki.fa + x / nd) * n + ki.fb + x % nd];
+    const auto PJ = gptr(md.PJ);              // Coulomb density (zeros in an exchange-only pass)
+    for (int x = lane; x < na * nb; x += D::NT) Dab[x] = PJ[(bi.fa + x / nb) * n + bi.fb + x % nb];
+    for (int x = lane; x < nc * nd; x += D::NT) Dcd[x] = PJ[(ki.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < na * nc; x += D::NT) Dac[x] = P[(bi.fa + x / nc) * n + ki.fa + x % nc];
     for (int x = lane; x < na * nd; x += D::NT) Dad[x] = P[(bi.fa + x / nd) * n + ki.fb + x % nd];
     for (int x = lane; x < nb * nc; x += D::NT) Dbc[x] = P[(bi.fb + x / nc) * n + ki.fa + x % nc];

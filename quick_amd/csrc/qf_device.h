@@ -484,7 +484,8 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     const int n = md.nbasis, ns = md.nshell;
     const auto cmc = cptr(md.cutmat);
     const auto cm = gptr(md.cutmat);
-    const auto Pc = cptr(md.P);
+    const auto PJc = cptr(md.PJ);
+    const auto PJ = gptr(md.PJ);
     const auto P = gptr(md.P);
     const auto bperm = cptr(bb.bperm);
 
@@ -502,7 +503,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
     for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = P[(ki.fa + c) * n + ki.fb + d];
+        for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = PJ[(ki.fa + c) * n + ki.fb + d];
 #pragma unroll
     for (int a = 0; a < na; ++a) {
 #pragma unroll
@@ -567,7 +568,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
-            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = Pc[(bi.fa + a) * n + bi.fb + b];
+            for (int b = 0; b < nb; ++b) Dab[a * nb + b] = PJc[(bi.fa + a) * n + bi.fb + b];
 #pragma unroll
         for (int b = 0; b < nb; ++b) {
 #pragma unroll

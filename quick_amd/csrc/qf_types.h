@@ -44,6 +44,8 @@ struct PairBlock {
 
 struct MolDesc {
     const double* P;        // nbasis x nbasis density, row-major
+    const double* PJ;       // density of the Coulomb terms: P, or zeros in the exchange-only
+                            // pass of an open-shell class built in two passes
     double* G;              // nbasis x nbasis accumulator (G = Gt + Gt^T afterwards)
     const double* cutmat;   // nshell x nshell max |P| per original shell pair
     int nbasis, nshell;

@@ -241,6 +241,10 @@ class FockEngine:
         """Primitive piece size of one class: 0 engine default, < 0 never split."""
         L.check(self.lib.qf_set_class_piece(self.h, int(cb), int(ck), int(piece)), "qf_set_class_piece")
 
+    def set_class_os(self, cb, ck, mode=-1):
+        """Open-shell form of class (cb|ck): 0/-1 fused, 1 two closed-shell passes."""
+        L.check(self.lib.qf_set_class_os(self.h, int(cb), int(ck), int(mode)), "qf_set_class_os")
+
     def set_timing(self, on=True):
         L.check(self.lib.qf_set_timing(self.h, 1 if on else 0), "qf_set_timing")
 

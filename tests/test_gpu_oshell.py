@@ -37,9 +37,16 @@ def _set_mode(eng, mode):
         for cb in range(6):
             for ck in range(cb + 1):
                 eng.set_class_tuning(cb, ck, 2, 0, 0, 0)
+    elif mode.startswith("two-pass"):
+        # every class as two closed-shell launches (exchange-only second pass)
+        for cb in range(6):
+            for ck in range(cb + 1):
+                eng.set_class_os(cb, ck, 1)
+                if mode == "two-pass-direct":
+                    eng.set_class_tuning(cb, ck, 0, 0, 0, 0)
 
 
-@pytest.mark.parametrize("mode", ["default", "direct", "coop"])
+@pytest.mark.parametrize("mode", ["default", "direct", "coop", "two-pass", "two-pass-direct"])
 @pytest.mark.parametrize("name,basis,mult,cutoff", CASES)
 def test_open_shell_fock_matches_oracle(name, basis, mult, cutoff, mode):
     announce(f"open-shell fock {name} {basis} mult={mult} {mode}")
