@@ -525,7 +525,7 @@ def coop_source(name, La, Lb, Lc, Ld):
 # (pp|ss) 4 waves 0.35 -> 0.32, (ss|ss) 5 waves 1.31 -> 1.23.  Tried and
 # slower: (ps|ps), (dp|ss), (dp|ds) at 2; (pp|ps) at 3; (ds|pp) at 2;
 # (ss|ss) at 6, (ps|ss) at 5, (ds|ss) at 4 (they spill).
-MIN_WAVES_OVERRIDE = {(2, 0, 1, 0): 2, (2, 0, 2, 0): 1, (1, 1, 0, 0): 4, (0, 0, 0, 0): 5}
+MIN_WAVES_OVERRIDE = {(2, 0, 1, 0): 2, (2, 0, 2, 0): 1, (1, 1, 0, 0): 4, (0, 0, 0, 0): 5, (1, 0, 0, 0): 4}
 # the open-shell kernels carry a second set of exchange accumulators and
 # density blocks: one wave fewer than the closed-shell kernel by default
 MIN_WAVES_OS_OVERRIDE = {(1, 0, 0, 0): 3}

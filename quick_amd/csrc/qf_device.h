@@ -80,6 +80,20 @@ __host__ __device__ __forceinline__ double qf_rcp(double x)
 #endif
 }
 
+// 1/sqrt(x) for normal x > 0: the hardware reciprocal square root refined
+// by two Newton steps r += r (1 - x r^2) / 2 (within an ulp); x = 0 gives
+// NaN, which only the branches that do not use it see.
+__host__ __device__ __forceinline__ double qf_rsqrt_fine(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rsq(x);
+    r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+    return fma(0.5 * r, fma(-x * r, r, 1.0), r);
+#else
+    return 1.0 / std::sqrt(x);
+#endif
+}
+
 // e^-x for x >= 0.  Cody-Waite reduction x = n ln2 + r, |r| <= ln2/2, with
 // ln2 split into a high part whose product with n is exact (|n| < 2^20) and
 // a low part (the fdlibm split); e^-r from its Taylor series to r^13
@@ -112,25 +126,6 @@ __host__ __device__ __forceinline__ double qf_exp_neg(double x)
 #endif
 }
 
-// sqrt(y) for normal y > 0 below 2^1000: the hardware reciprocal square
-// root refined by one Goldschmidt iteration and two Newton corrections of
-// the root (no range scaling, which only denormal or huge inputs need).
-__host__ __device__ __forceinline__ double qf_sqrt_pos(double y)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double r = __builtin_amdgcn_rsq(y);
-    double g = y * r, h = 0.5 * r;
-    const double e = fma(-h, g, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    g = fma(fma(-g, g, y), h, g);
-    g = fma(fma(-g, g, y), h, g);
-    return g;
-#else
-    return std::sqrt(y);
-#endif
-}
-
 template <typename R>
 struct VrrPar {
     R PA[3], WP[3], QC[3], WQ[3];
@@ -152,8 +147,14 @@ struct VrrPar {
 // the padded evaluation is bit-identical to the unpadded one).  Intervals
 // 0-2: F1 = p(u), u = X, X-2, X-4, and F_0 = 2X F1 + e^-X; intervals 3-6:
 // u = 1/X and F_0 = p(u) e^-X + sqrt(pi/4/X).
-constexpr int kBoysRows = 7, kBoysCols = 12;
-__host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
+// After the rows: the interval of X by ceil(X) - 1 (clamped to 0..33), as
+// a double k, so that the interval is one table read instead of six
+// comparisons: X <= 1 -> 0, (1, 3] -> 1, (3, 5] -> 2, (5, 10] -> 3,
+// (10, 15] -> 4, (15, 33] -> 5, > 33 -> 6 (X > b  <=>  ceil(X) - 1 >= b for
+// the integer bounds b).
+constexpr int kBoysRows = 7, kBoysCols = 12, kBoysMap = kBoysRows * kBoysCols, kBoysMapN = 34;
+constexpr int kBoysTab = kBoysMap + kBoysMapN;
+__host__ __device__ constexpr double kBoysFit[kBoysTab] = {
     0.0, 0.0, -8.36313918003957e-08, 1.21222603512827e-06, -1.15662609053481e-05, 9.25197374512647e-05,
     -6.40994113129432e-04, 3.78787044215009e-03, -1.85185172458485e-02, 7.14285713298222e-02,
     -1.99999999997023e-01, 3.33333333333318e-01,
@@ -169,16 +170,18 @@ __host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
     -4.9893752514047e-01, -2.1916512131607e-05,
     0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.9623264149430e-01, -4.9695241464490e-01,
     -6.0156581186481e-05,
-    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+    // interval map, ceil(X) - 1 = 0 .. 33
+    0, 1, 1, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 4, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 6};
 
 // The kernels read the table from LDS: per-lane reads of a tiny table, and
 // no fit constant is held in a register across the primitive loops.
-#define QF_BOYS_LDS_DECL __shared__ double qf_boys_tab[kBoysRows * kBoysCols]
-#define QF_BOYS_LDS_FILL()                                                       \
-    do {                                                                         \
-        for (int _x = threadIdx.x; _x < kBoysRows * kBoysCols; _x += blockDim.x) \
-            qf_boys_tab[_x] = kBoysFit[_x];                                      \
-        __syncthreads();                                                         \
+#define QF_BOYS_LDS_DECL __shared__ double qf_boys_tab[kBoysTab]
+#define QF_BOYS_LDS_FILL()                                              \
+    do {                                                                \
+        for (int _x = threadIdx.x; _x < kBoysTab; _x += blockDim.x)     \
+            qf_boys_tab[_x] = kBoysFit[_x];                             \
+        __syncthreads();                                                \
     } while (0)
 #define QF_BOYS_LDS_PTR ((const QF_AS_LDS double*)qf_boys_tab)
 
@@ -188,18 +191,22 @@ __host__ __device__ constexpr double kBoysFit[kBoysRows * kBoysCols] = {
 template <int L, typename Tab, int EXTRA = 3>
 __host__ __device__ __forceinline__ void boys_quick(const double X, const int maxm, double (&F)[L + 1], Tab tab)
 {
-    const double XINV = qf_rcp(X);
+    // 1/sqrt(X) once: 1/X is its square and sqrt(pi/4X) a multiple of it
+    const double RS = qf_rsqrt_fine(X);
+    const double XINV = RS * RS;
     const bool up = (X > 1.0e-1) || (X > 1.0e-4 && maxm < 4);
-    // interval index without branches; both forms of F0 evaluated and one
-    // selected (measured on (H2O)32: 7.6 % faster than branching and the
-    // library exp / sqrt)
+    // interval index from the map (no branches); both forms of F0 evaluated
+    // and one selected (measured on (H2O)32: 7.6 % faster than branching and
+    // the library exp / sqrt)
     const double E = qf_exp_neg(X);
-    const int k = int(X > 1.0) + int(X > 3.0) + int(X > 5.0) + int(X > 10.0) + int(X > 15.0) + int(X > 33.0);
-    const double u = k >= 3 ? XINV : fma(-2.0, double(k), X);
+    const int ci = (int)fmin(fmax(ceil(X) - 1.0, 0.0), double(kBoysMapN - 1));
+    const double kd = tab[kBoysMap + ci];
+    const int k = (int)kd;
+    const double u = kd >= 3.0 ? XINV : fma(-2.0, kd, X);
     double p = 0.0;
 #pragma unroll
     for (int c = 0; c < kBoysCols; ++c) p = fma(p, u, tab[k * kBoysCols + c]);
-    const double WW1 = k >= 3 ? fma(p, E, qf_sqrt_pos(0.78539816339744830962 * XINV)) : fma(X + X, p, E);
+    const double WW1 = k >= 3 ? fma(p, E, 0.88622692545275801365 * RS) : fma(X + X, p, E);
     if (up) {
         F[0] = WW1;
         const double h = 0.5 * XINV;
