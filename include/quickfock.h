@@ -83,8 +83,9 @@ int qf_set_cutoffs(qf_engine *e, double integral_cutoff, double prim_limit, doub
  * differences and the Fock accumulation stay FP64). */
 int qf_set_precision(qf_engine *e, int mode, double split);
 /* Which classes a mixed build splits (native tuning, no QUICK counterpart):
- * those with La+Lb+Lc+Ld >= min_lsum that have per-lane kernels (default 0:
- * all of them); the others run every quartet in FP64. */
+ * those with La+Lb+Lc+Ld >= min_lsum that have per-lane kernels; -1 (the
+ * default): 0 (all of them) above 256 basis functions, 5 up to 256.  The
+ * others run every quartet in FP64. */
 int qf_set_split_classes(qf_engine *e, int min_lsum);
 
 /* Schwarz bounds.  Pass NULL to have them computed on the GPU, or QUICK's

@@ -253,7 +253,7 @@ struct qf_engine {
     double list_cutoff() const { return grad_cutoff > 0.0 ? std::min(cutoff, grad_cutoff) : cutoff; }
     int prec = QF_PREC_FP64;
     double split = 1e-4;
-    int split_lmin = 0;      // classes with La+Lb+Lc+Ld >= split_lmin are split in mixed builds
+    int split_lmin = -1;     // classes with La+Lb+Lc+Ld >= split_lmin are split in mixed builds (-1: by size)
     double split_thr = 0.0;  // Ycut_ij*Ycut_kl below which quartets run in FP32
     int replicas = 8;        // copies of G the atomics are spread over
     int shuffle = 1;         // permute tasks inside a launch (decorrelates atomic targets)
@@ -722,8 +722,13 @@ int prepare(qf_engine* e)
                 // FP64 Boys function is most of a light class's work) and halves
                 // the registers of the spilling d classes.
                 const int Lsum_s = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
-                const bool split_class =
-                    e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= e->split_lmin;
+                // By default every class above 256 functions, and only the
+                // classes with La+Lb+Lc+Ld >= 5 below: a small molecule's
+                // light classes hold too few quartets for the second launch
+                // to pay (256 batched molecules: 1389 builds/s splitting all,
+                // FP64 1432).
+                const int lmin = e->split_lmin >= 0 ? e->split_lmin : e->nbasis <= 256 ? 5 : 0;
+                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= lmin;
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
@@ -1248,7 +1253,7 @@ int qf_set_precision(qf_engine* e, int mode, double split)
 
 int qf_set_split_classes(qf_engine* e, int min_lsum)
 {
-    if (!e || min_lsum < 0) return fail(QF_ERR_ARG, "min_lsum must be >= 0");
+    if (!e || min_lsum < -1) return fail(QF_ERR_ARG, "min_lsum must be >= -1");
     e->split_lmin = min_lsum;
     e->prepared = false;
     return QF_OK;

@@ -73,8 +73,8 @@ class FockEngine:
         self.precision = precision
         L.check(self.lib.qf_set_precision(self.h, mode, float(split)), "qf_set_precision")
 
-    def set_split_classes(self, min_lsum=0):
-        """Mixed builds split the classes with La+Lb+Lc+Ld >= min_lsum."""
+    def set_split_classes(self, min_lsum=-1):
+        """Mixed builds split the classes with La+Lb+Lc+Ld >= min_lsum (-1: by molecule size)."""
         L.check(self.lib.qf_set_split_classes(self.h, int(min_lsum)), "qf_set_split_classes")
 
     def schwarz(self):
