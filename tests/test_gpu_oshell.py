@@ -63,6 +63,28 @@ def test_open_shell_fock_matches_oracle(name, basis, mult, cutoff, mode):
     assert np.max(np.abs(Ga - Ga.T)) <= 1e-14 * max(1.0, np.max(np.abs(Ga)))
 
 
+@pytest.mark.parametrize("name,basis", [("water+", "cc-pVTZ"), ("water+", "def2-TZVP")])
+def test_open_shell_f_shells_match_oracle(name, basis):
+    """f classes in the open-shell pass: the wave-per-quartet kernel reloads
+    the exchange density blocks from P_a - P_b (the (ff|..) classes keep
+    their node-program slots in global memory), fused and two-pass forms."""
+    announce(f"open-shell f shells {name} {basis}")
+    mol = molecule(name)
+    orc = oracle_for(mol, basis)
+    Pa, Pb = seeded_spin_densities_for(orc, mol, 2)
+    Ga_ref, Gb_ref, nq = orc.fock2e_os(Pa, Pb, cutoff=1e-11, threads=4)
+    for two in (False, True):
+        eng = _engine(product_basis(mol, basis), cutoff=1e-11)
+        if two:
+            for cb in range(10):
+                for ck in range(cb + 1):
+                    eng.set_class_os(cb, ck, 1)
+        Ga, Gb = eng.fock2e_os(Pa, Pb)
+        ea, eb = float(np.max(np.abs(Ga - Ga_ref))), float(np.max(np.abs(Gb - Gb_ref)))
+        print(f"  two-pass={two}: max|dFa| = {ea:.3e}  max|dFb| = {eb:.3e}  (quartets {nq})", flush=True)
+        assert ea <= FOCK_TOL and eb <= FOCK_TOL
+
+
 def test_open_shell_closed_shell_limit():
     """Pa = Pb = P/2 gives Ga = Gb = the closed-shell G(P)."""
     announce("open-shell closed-shell limit water3 6-31G*")
