@@ -749,7 +749,8 @@ int prepare(qf_engine* e)
                 // wave-per-quartet kernel a window (the class tuning's
                 // tasks_per_group): small windows give more workgroups to
                 // classes with few quartets
-                const int win = coop ? (ct.tpg > 0 ? std::min(ct.tpg, qf::WAVE) : kCoopWindow) : qf::WAVE;
+                const int coop_win = e->nbasis <= 256 ? 1 : kCoopWindow;
+                const int win = coop ? (ct.tpg > 0 ? std::min(ct.tpg, qf::WAVE) : coop_win) : qf::WAVE;
                 size_t lds = 0, lds_os = 0;
                 long long group_tasks = 0;
                 for (size_t bbk = 0; bbk < e->blocks.size(); ++bbk) {

@@ -103,17 +103,20 @@ def test_primitive_pieces_match_oracle(piece, mode):
     P, _ = seeded_density_for(orc, mol)
     G_ref, _ = orc.fock2e(P, cutoff=1e-12, threads=4)
     bs = product_basis(mol, "6-31G*")
-    eng = _engine(bs, cutoff=1e-12)
-    eng.set_prim_split(-1)
+
+    def engine(split):
+        e = _engine(bs, cutoff=1e-12)
+        e.set_prim_split(split)
+        if mode == "direct":
+            e.set_tile(False)
+        elif mode == "coop":
+            for cb in range(6):
+                for ck in range(cb + 1):
+                    e.set_class_tuning(cb, ck, 2, 0, 0, 0)
+        return e
+    eng = engine(-1)            # whole quartets, same kernels
     G_whole = eng.fock2e(P)
-    eng2 = _engine(bs, cutoff=1e-12)
-    eng2.set_prim_split(piece)
-    if mode == "direct":
-        eng2.set_tile(False)
-    elif mode == "coop":
-        for cb in range(6):
-            for ck in range(cb + 1):
-                eng2.set_class_tuning(cb, ck, 2, 0, 0, 0)
+    eng2 = engine(piece)
     G = eng2.fock2e(P)
     assert eng2.stats()["n_tasks_fp64"] > eng.stats()["n_tasks_fp64"]
     err, dw = float(np.max(np.abs(G - G_ref))), float(np.max(np.abs(G - G_whole)))
