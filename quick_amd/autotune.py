@@ -22,7 +22,7 @@ CANDIDATES = [
     (0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 0, 2), (0, 0, 0, 4),
     (1, 4, 16, 0), (1, 4, 8, 0), (1, 2, 8, 0), (1, 1, 4, 0), (1, 4, 32, 0),
     (1, 4, 16, 2), (1, 4, 16, 4), (1, 4, 32, 2), (1, 4, 32, 4),
-    (2, 0, 0, 0), (2, 0, 0, 1), (2, 0, 0, 2), (2, 0, 2, 1), (2, 0, 4, 1), (2, 0, 16, 1),
+    (2, 0, 0, 0), (2, 0, 0, 1), (2, 0, 0, 2), (2, 0, 1, 1), (2, 0, 2, 1), (2, 0, 4, 1), (2, 0, 16, 1),
 ]
 
 
