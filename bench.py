@@ -10,7 +10,7 @@ F += G), replayed from a HIP graph.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|mixed]
 
-Prints one JSON line.  Beside the headline it carries, under ``north_star``,
+Prints one JSON line.  Beside the headline it carries, under ``workloads``,
 the other workloads of the metric measured in the same run:
 * ``split_precision``: (Gly)10 6-31G(d,p) SCF and builds/s in FP64 and in
   mixed precision;
@@ -90,8 +90,8 @@ def main():
                     help="primitive pairs per task piece (0: automatic, < 0: never split)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
-    ap.add_argument("--no-north-star", action="store_true",
-                    help="headline workload only (no split-precision, batched and serving sub-measurements)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="headline workload and gradient only (no split-precision, batched, serving and open-shell measurements)")
     ap.add_argument("--serve-rate", type=float, default=10.0, help="default run: serving arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=40, help="default run: serving requests")
     a = ap.parse_args()
@@ -131,7 +131,7 @@ def main():
         return
     out, ctx = measure_h2o32(a, dev)
     ns = None
-    if not a.no_north_star:
+    if not a.headline_only:
         c3, ctx3 = measure_gly10(a, dev)
         ns = {"split_precision": c3, "batched": measure_batch(a, dev), "serving": measure_serve(a, dev),
               "open_shell": measure_uhf(a, dev)}
@@ -145,7 +145,7 @@ def main():
         if ns is not None and "c3" in checks:
             ns["split_precision"]["vs_fp64_oracle"] = checks["c3"]
         out["vs_fp64_oracle"] = {k[3:]: v for k, v in checks.items() if k.startswith("c2_")}
-    out["north_star"] = ns
+    out["workloads"] = ns
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

@@ -17,9 +17,11 @@
  *                                 Head-Gordon-Pople recursion restated here
  *   copySym ..................... src/subs/copySym.f90
  *
- * The product path never links this file; it is the checker.
+ * The product path never links this file; it is test infrastructure.
  *
- * Parity status of each function (what pins it) is listed in DESIGN.md.
+ * Every function is pinned by a check in tests/test_oracle.py (closed forms,
+ * an independent McMurchie-Davidson code, brute-force J/K, finite
+ * differences, QUICK's saved SCF energies).
  */
 #include "quick_oracle.h"
 
