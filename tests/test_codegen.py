@@ -126,3 +126,36 @@ def test_coop_node_programs_match_recursion(cls):
     assert np.all(np.isfinite(got))
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
     assert t["total"] * 8 <= G.COOP_LDS_CAP or t["glob"]
+
+
+def test_fp32_boys_grid_table():
+    """The FP32 Boys grid (gen_boys.py, read by boys_f32 for the FP32 quartets
+    of split-precision builds): the committed header equals the generator,
+    the series values equal F_m(x) = Gamma(m+1/2) P(m+1/2, x) / (2 x^(m+1/2))
+    (scipy's regularized incomplete gamma) to 1e-13, and the 5-term Taylor
+    step the kernel takes from the nearest grid point stays below 1e-7
+    relative over the grid range for every L the per-lane classes use."""
+    import math
+    import os
+    from scipy.special import gamma, gammainc
+    from quick_amd.codegen import gen_boys as GB
+    hdr = os.path.join(os.path.dirname(G.__file__), "..", "csrc", "gen", "boys_grid.h")
+    assert open(hdr).read() == GB.header()
+    tab = GB.table()
+    worst = 0.0
+    for i in range(1, GB.NG, 7):
+        x = i * GB.H
+        for m in range(GB.MG):
+            ref = gamma(m + 0.5) * gammainc(m + 0.5, x) / (2.0 * x ** (m + 0.5))
+            worst = max(worst, abs(tab[i][m] - ref) / ref)
+    assert worst < 1e-13
+    assert all(abs(tab[0][m] - 1.0 / (2 * m + 1)) < 1e-16 for m in range(GB.MG))
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for L in range(6):
+        for x in rng.uniform(0.0, (GB.NG - 1) * GB.H - 1e-9, 400):
+            i = int(x / GB.H + 0.5)
+            d = i * GB.H - x
+            s = sum(tab[i][L + k] * d ** k / math.factorial(k) for k in range(5))
+            worst = max(worst, abs(s - GB.boys_series(L, x)) / GB.boys_series(L, x))
+    assert worst < 1e-7
