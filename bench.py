@@ -707,7 +707,7 @@ def measure_uhf(a, dev):
     Fa, Fb, F = torch.zeros_like(Pa), torch.zeros_like(Pa), torch.zeros_like(Pa)
     if a.autotune:
         # tuned on the open-shell build itself (its kernels carry more state)
-        tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32+", builds=2, os_modes=(0, 1),
+        tune_or_load(e, sh, retune=a.autotune == 2, save_path=a.tune_out, tag="h2o32+", builds=2, os_modes=(0, 1, 2),
                      build=lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh))
     t_os = _timed(lambda: e.fock2e_os_device(Pa.data_ptr(), Pb.data_ptr(), Fa.data_ptr(), Fb.data_ptr(), sh),
                   a.steps, a.warmup)

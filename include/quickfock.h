@@ -254,11 +254,15 @@ int qf_set_prim_split(qf_engine *e, int piece);
 /* Per-class primitive piece size (class indices as qf_set_class_tuning):
  * 0 follows qf_set_prim_split, < 0 never splits the class. */
 int qf_set_class_piece(qf_engine *e, int cb, int ck, int piece);
-/* Open-shell form of a class (native tuning, no QUICK counterpart): 0 (and
- * -1, the default) one fused launch that digests each integral with both
- * densities; 1 two closed-shell launches, the total density into G and the
- * spin density, exchange terms only, into the second accumulator (each
- * integral computed twice, but no second set of exchange state per lane). */
+/* Open-shell form of a class (native tuning, no QUICK counterpart): 0 one
+ * fused launch that digests each integral with both densities, the second
+ * exchange's per-task terms held in registers; 2 (and -1, the default) the
+ * same, lean: on the LDS-tile kernel (FP64) those terms are re-read and
+ * added to the tile every quartet instead, so the integral loop keeps the
+ * closed-shell register count (other kernels take form 0); 1 two
+ * closed-shell launches, the total density into G and the spin density,
+ * exchange terms only, into the second accumulator (each integral computed
+ * twice, but no second set of exchange state per lane). */
 int qf_set_class_os(qf_engine *e, int cb, int ck, int mode);
 
 /* HIP-event timing of every class launch (on the build stream); read sums

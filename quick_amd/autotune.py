@@ -60,10 +60,10 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
     density to build with, or ``build()`` runs one build of the kind to tune
     (e.g. the open-shell build).  Batch: ``densities`` holds one host density
     per engine.  Returns {class name: (candidate, ms)} (candidate = shape +
-    (piece,)) and leaves the best settings applied.  ``os_modes`` (e.g. (0, 1)
-    with an open-shell ``build``): a third pass choosing each class's
-    open-shell form (fused or two closed-shell passes), appended to the
-    candidate."""
+    (piece,)) and leaves the best settings applied.  ``os_modes`` (e.g.
+    (0, 1, 2) with an open-shell ``build``): a third pass choosing each
+    class's open-shell form (fused, two closed-shell passes, fused lean),
+    appended to the candidate."""
     from .engine import FockBatch
     single = not isinstance(engines, (list, tuple))
     engs = [engines] if single else list(engines)

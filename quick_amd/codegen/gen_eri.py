@@ -622,12 +622,15 @@ def launcher_source(name, diag, coop=False, lane=True):
                 "        hipLaunchKernelGGL((fock_kernel<%s, double>), dim3(grid), dim3(WAVE), 0, s, t, nt, b, m);" % name,
                 "    return 0;",
                 "}", "",
-                "// os: 0 closed shell, 1 open shell with a second LDS tile, 2 open shell",
-                "// with the second exchange terms straight to G2",
+                "// os & 3: 0 closed shell, 1 open shell with a second LDS tile, 2 open",
+                "// shell with the second exchange terms straight to G2; os & 4: the lean",
+                "// open-shell form (FP64; FP32 launches take the register form)",
                 "void launch_fock_tile_%s(const TileGroup* g, int ng, const Task* t, const PairBlock* b,"
                 " const MolDesc* m, int fp32, int os, int waves, size_t lds, int grid, hipStream_t s)" % name, "{",
                 "    if (ng <= 0) return;",
                 "    if (grid <= 0 || grid > ng) grid = ng;",
+                "    const bool lean = (os & 4) != 0;",
+                "    os &= 3;",
                 "    const int k2 = os == 1;",
                 "    if (fp32 && os)",
                 "        hipLaunchKernelGGL((fock_tile_kernel<%s, float, true>), dim3(grid), dim3(waves * WAVE), lds, s,"
@@ -635,6 +638,9 @@ def launcher_source(name, diag, coop=False, lane=True):
                 "    else if (fp32)",
                 "        hipLaunchKernelGGL((fock_tile_kernel<%s, float>), dim3(grid), dim3(waves * WAVE), lds, s,"
                 " g, ng, t, b, m, 0);" % name,
+                "    else if (os && lean)",
+                "        hipLaunchKernelGGL((fock_tile_kernel<%s, double, true, true>), dim3(grid), dim3(waves * WAVE), lds,"
+                " s, g, ng, t, b, m, k2);" % name,
                 "    else if (os)",
                 "        hipLaunchKernelGGL((fock_tile_kernel<%s, double, true>), dim3(grid), dim3(waves * WAVE), lds, s,"
                 " g, ng, t, b, m, k2);" % name,

@@ -266,7 +266,7 @@ struct qf_engine {
     struct ClassTune {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
         int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
-        int os = -1;                                    // open-shell form: -1 default (fused), 0 fused, 1 two passes
+        int os = -1;   // open-shell form: -1 default (2), 0 fused, 1 two passes, 2 fused lean (tile kernels)
     };
     ClassTune ctune[kNPC][kNPC];
     // pair blocks
@@ -280,6 +280,7 @@ struct qf_engine {
         size_t lds;
         size_t lds_os;   // open-shell pass with a second exchange tile (0: it does not fit)
         int os_split;    // open-shell builds run this class as two closed-shell passes
+        int os_lean;     // ... or, on the tile kernel, in the fused lean form
     };
     std::vector<Launch> launches;
     bool prepared = false;
@@ -885,6 +886,7 @@ int prepare(qf_engine* e)
                     L.lds = lds;
                     L.lds_os = lds_os <= kMaxTileLds ? lds_os : 0;
                     L.os_split = ct.os == 1;
+                    L.os_lean = ct.os == 2 || ct.os < 0;
                     e->launches.push_back(L);
                 }
                 if (fp32) e->stats.n_tasks_fp32 += count; else e->stats.n_tasks_fp64 += count;
@@ -1072,8 +1074,8 @@ int build_pass(qf_engine* e, const double* dP, const double* dcm, double* dF, in
                 if (!ft) return fail(QF_ERR_ARG, "no tile kernel for class");
                 // open shell: a second exchange tile where two fit, else its terms go to Gt2
                 const int tos = !los ? 0 : L.lds_os ? 1 : 2;
-                ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, m, L.fp32, tos, L.waves,
-                   tos == 1 ? L.lds_os : L.lds, e->persist, s);
+                ft(e->d_groups.p + L.goffset, L.gcount, e->d_tasks.p, e->d_blocks.p, m, L.fp32,
+                   tos | (los && L.os_lean ? 4 : 0), L.waves, tos == 1 ? L.lds_os : L.lds, e->persist, s);
             } else {
                 qf::FockLaunch f = qf::fock_launcher(L.cb, L.ck);
                 if (!f) return fail(QF_ERR_ARG, "no per-lane kernel for class");
@@ -1676,7 +1678,7 @@ int qf_set_class_os(qf_engine* e, int cb, int ck, int mode)
 {
     if (!e) return fail(QF_ERR_ARG, "NULL");
     if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
-    if (mode < -1 || mode > 1) return fail(QF_ERR_ARG, "mode must be -1, 0 (fused) or 1 (two passes)");
+    if (mode < -1 || mode > 2) return fail(QF_ERR_ARG, "mode must be -1, 0 (fused), 1 (two passes) or 2 (fused lean)");
     e->ctune[cb][ck].os = mode;
     e->prepared = false;
     return QF_OK;

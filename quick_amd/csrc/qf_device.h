@@ -476,8 +476,11 @@ struct TileRef {
 };
 
 // OS = true (the fused open-shell pass): every integral is digested a second
-// time into exchange terms of md.P2 that go to G2 (or the tile k2).
-template <class C, typename R, bool TILE, bool OS>
+// time into exchange terms of md.P2 that go to G2 (or the tile k2).  LEAN
+// (tile kernels): the second exchange keeps no per-task state in registers;
+// its a-row density blocks are re-read and its a-row terms added to the tile
+// every quartet, so the contraction runs at the closed-shell register count.
+template <class C, typename R, bool TILE, bool OS, bool LEAN = false>
 __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, const PairBlock& kb,
                                           const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_GLOBAL double* G2,
                                           const TileRef& tr)
@@ -504,7 +507,9 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     const double cm_ad = cm[sa * ns + ki.sb];
 
     const auto P2 = gptr(md.P2);
-    constexpr int os2 = OS ? 1 : 0;          // sizes of the second-density arrays (1 when unused)
+    constexpr bool kLean = OS && TILE && LEAN;
+    constexpr int os2 = (OS && !kLean) ? 1 : 0;   // sizes of the per-task second-density arrays (1 when unused)
+    constexpr int osq = OS ? 1 : 0;               // the same for the per-quartet arrays
     double Dcd[nc * nd], Dac[na * nc], Dad[na * nd];
     double Dac2[os2 * na * nc + 1 - os2], Dad2[os2 * na * nd + 1 - os2];
 #pragma unroll
@@ -517,7 +522,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         for (int c = 0; c < nc; ++c) Dac[a * nc + c] = P[(fa + a) * n + ki.fa + c];
 #pragma unroll
         for (int d = 0; d < nd; ++d) Dad[a * nd + d] = P[(fa + a) * n + ki.fb + d];
-        if constexpr (OS) {
+        if constexpr (OS && !kLean) {
 #pragma unroll
             for (int c = 0; c < nc; ++c) Dac2[a * nc + c] = P2[(fa + a) * n + ki.fa + c];
 #pragma unroll
@@ -532,7 +537,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
     for (int k = 0; k < na * nc; ++k) Kac[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < na * nd; ++k) Kad[k] = 0.0;
-    if constexpr (OS) {
+    if constexpr (OS && !kLean) {
 #pragma unroll
         for (int k = 0; k < na * nc; ++k) Kac2[k] = 0.0;
 #pragma unroll
@@ -584,8 +589,11 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int d = 0; d < nd; ++d) Dbd[b * nd + d] = P[(bi.fb + b) * n + ki.fb + d];
         }
         double Jab[na * nb], Kbc[nb * nc], Kbd[nb * nd];
-        double Dbc2[os2 * nb * nc + 1 - os2], Dbd2[os2 * nb * nd + 1 - os2];
-        double Kbc2[os2 * nb * nc + 1 - os2], Kbd2[os2 * nb * nd + 1 - os2];
+        double Dbc2[osq * nb * nc + 1 - osq], Dbd2[osq * nb * nd + 1 - osq];
+        double Kbc2[osq * nb * nc + 1 - osq], Kbd2[osq * nb * nd + 1 - osq];
+        constexpr int lq = kLean ? 1 : 0;
+        double Dac2q[lq * na * nc + 1 - lq], Dad2q[lq * na * nd + 1 - lq];
+        double Kac2q[lq * na * nc + 1 - lq], Kad2q[lq * na * nd + 1 - lq];
 #pragma unroll
         for (int k = 0; k < na * nb; ++k) Jab[k] = 0.0;
 #pragma unroll
@@ -605,6 +613,23 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
             for (int k = 0; k < nb * nd; ++k) Kbd2[k] = 0.0;
         }
+        if constexpr (kLean) {
+            // opaque offsets: the blocks are the same every quartet, and a
+            // hoisted load would hold them in registers across the loop again
+            int oc = fa * n + ki.fa, od = fa * n + ki.fb;
+            asm volatile("" : "+v"(oc), "+v"(od));
+#pragma unroll
+            for (int a = 0; a < na; ++a) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) Dac2q[a * nc + c] = P2[oc + a * n + c];
+#pragma unroll
+                for (int d = 0; d < nd; ++d) Dad2q[a * nd + d] = P2[od + a * n + d];
+            }
+#pragma unroll
+            for (int k = 0; k < na * nc; ++k) Kac2q[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < na * nd; ++k) Kad2q[k] = 0.0;
+        }
 #pragma unroll
         for (int a = 0; a < na; ++a)
 #pragma unroll
@@ -621,7 +646,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         Kad[a * nd + d] += Dbc[b * nc + c] * v;
                         Kbc[b * nc + c] += Dad[a * nd + d] * v;
                         Kbd[b * nd + d] += Dac[a * nc + c] * v;
-                        if constexpr (OS) {
+                        if constexpr (kLean) {
+                            Kac2q[a * nc + c] += Dbd2[b * nd + d] * v;
+                            Kad2q[a * nd + d] += Dbc2[b * nc + c] * v;
+                            Kbc2[b * nc + c] += Dad2q[a * nd + d] * v;
+                            Kbd2[b * nd + d] += Dac2q[a * nc + c] * v;
+                        } else if constexpr (OS) {
                             Kac2[a * nc + c] += Dbd2[b * nd + d] * v;
                             Kad2[a * nd + d] += Dbc2[b * nc + c] * v;
                             Kbc2[b * nc + c] += Dad2[a * nd + d] * v;
@@ -668,6 +698,25 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         }
                     }
                 }
+                if constexpr (kLean) {
+#pragma unroll
+                    for (int a = 0; a < na; ++a) {
+                        if (tr.k2) {
+                            QF_AS_LDS double* row2 = tr.k2 + a * tr.ncol;
+#pragma unroll
+                            for (int c = 0; c < nc; ++c) lds_add(&row2[ki.cola + c], Kac2q[a * nc + c]);
+#pragma unroll
+                            for (int d = 0; d < nd; ++d) lds_add(&row2[ki.colb + d], Kad2q[a * nd + d]);
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < nc; ++c)
+                                atomic_add(&G2[(fa + a) * n + ki.fa + c], kscale * Kac2q[a * nc + c]);
+#pragma unroll
+                            for (int d = 0; d < nd; ++d)
+                                atomic_add(&G2[(fa + a) * n + ki.fb + d], kscale * Kad2q[a * nd + d]);
+                        }
+                    }
+                }
             } else {
 #pragma unroll
                 for (int b = 0; b < nb; ++b) {
@@ -702,7 +751,7 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kac[a * nc + c]);
 #pragma unroll
             for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kad[a * nd + d]);
-            if constexpr (OS) {
+            if constexpr (OS && !kLean) {
                 if (tr.k2) {
                     QF_AS_LDS double* row2 = tr.k2 + a * tr.ncol;
 #pragma unroll
@@ -768,9 +817,9 @@ void fock_kernel(const Task* __restrict__ tasks, int ntasks, const PairBlock* __
 // (na + nbra*nb) * ncol * (k2 ? 2 : 1) + nbra*na*nb doubles.  OS: the
 // second exchange terms go to a second LDS tile (k2 = 1) or straight to
 // G2 (k2 = 0, where two tiles do not fit).
-template <class C, typename R, bool OS = false>
+template <class C, typename R, bool OS = false, bool LEAN = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(WAVE, 4 * WAVE),
-                          amdgpu_waves_per_eu(OS ? C::MIN_WAVES_OS : C::MIN_WAVES)))
+                          amdgpu_waves_per_eu((OS && !LEAN) ? C::MIN_WAVES_OS : C::MIN_WAVES)))
 void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const Task* __restrict__ tasks,
                       const PairBlock* __restrict__ blocks, const MolDesc* __restrict__ mols, int k2)
 {
@@ -811,7 +860,7 @@ void fock_tile_kernel(const TileGroup* __restrict__ groups, int ngroups, const T
         const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
         for (int ti = g.task_begin + wave; ti < g.task_end; ti += nw) {
             const Task t = cptr(tasks)[wave_uniform(ti)];
-            fock_task<C, R, true, OS>(t, bb, kb, md, G, G2, tr);
+            fock_task<C, R, true, OS, LEAN>(t, bb, kb, md, G, G2, tr);
         }
         __syncthreads();
 

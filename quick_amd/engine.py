@@ -242,7 +242,9 @@ class FockEngine:
         L.check(self.lib.qf_set_class_piece(self.h, int(cb), int(ck), int(piece)), "qf_set_class_piece")
 
     def set_class_os(self, cb, ck, mode=-1):
-        """Open-shell form of class (cb|ck): 0/-1 fused, 1 two closed-shell passes."""
+        """Open-shell form of class (cb|ck): 0 fused, 2/-1 fused lean (LDS-tile
+        kernel: no per-task second-exchange state in registers), 1 two
+        closed-shell passes."""
         L.check(self.lib.qf_set_class_os(self.h, int(cb), int(ck), int(mode)), "qf_set_class_os")
 
     def set_timing(self, on=True):

@@ -37,6 +37,11 @@ def _set_mode(eng, mode):
         for cb in range(6):
             for ck in range(cb + 1):
                 eng.set_class_tuning(cb, ck, 2, 0, 0, 0)
+    elif mode == "register":
+        # the fused form with the second exchange's per-task terms in registers
+        for cb in range(6):
+            for ck in range(cb + 1):
+                eng.set_class_os(cb, ck, 0)
     elif mode.startswith("two-pass"):
         # every class as two closed-shell launches (exchange-only second pass)
         for cb in range(6):
@@ -46,7 +51,7 @@ def _set_mode(eng, mode):
                     eng.set_class_tuning(cb, ck, 0, 0, 0, 0)
 
 
-@pytest.mark.parametrize("mode", ["default", "direct", "coop", "two-pass", "two-pass-direct"])
+@pytest.mark.parametrize("mode", ["default", "register", "direct", "coop", "two-pass", "two-pass-direct"])
 @pytest.mark.parametrize("name,basis,mult,cutoff", CASES)
 def test_open_shell_fock_matches_oracle(name, basis, mult, cutoff, mode):
     announce(f"open-shell fock {name} {basis} mult={mult} {mode}")
@@ -67,21 +72,21 @@ def test_open_shell_fock_matches_oracle(name, basis, mult, cutoff, mode):
 def test_open_shell_f_shells_match_oracle(name, basis):
     """f classes in the open-shell pass: the wave-per-quartet kernel reloads
     the exchange density blocks from P_a - P_b (the (ff|..) classes keep
-    their node-program slots in global memory), fused and two-pass forms."""
+    their node-program slots in global memory), fused, fused lean and
+    two-pass forms."""
     announce(f"open-shell f shells {name} {basis}")
     mol = molecule(name)
     orc = oracle_for(mol, basis)
     Pa, Pb = seeded_spin_densities_for(orc, mol, 2)
     Ga_ref, Gb_ref, nq = orc.fock2e_os(Pa, Pb, cutoff=1e-11, threads=4)
-    for two in (False, True):
+    for form in (2, 0, 1):   # fused lean (default), fused, two passes
         eng = _engine(product_basis(mol, basis), cutoff=1e-11)
-        if two:
-            for cb in range(10):
-                for ck in range(cb + 1):
-                    eng.set_class_os(cb, ck, 1)
+        for cb in range(10):
+            for ck in range(cb + 1):
+                eng.set_class_os(cb, ck, form)
         Ga, Gb = eng.fock2e_os(Pa, Pb)
         ea, eb = float(np.max(np.abs(Ga - Ga_ref))), float(np.max(np.abs(Gb - Gb_ref)))
-        print(f"  two-pass={two}: max|dFa| = {ea:.3e}  max|dFb| = {eb:.3e}  (quartets {nq})", flush=True)
+        print(f"  form {form}: max|dFa| = {ea:.3e}  max|dFb| = {eb:.3e}  (quartets {nq})", flush=True)
         assert ea <= FOCK_TOL and eb <= FOCK_TOL
 
 
