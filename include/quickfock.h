@@ -128,9 +128,14 @@ int qf_fock_build_device_noalloc(qf_engine *e, const double *d_P, double *d_F, c
  * qf_set_grad_cutoff: QUICK's gradCutoff (default: the integral cutoff); a value below
  * the integral cutoff makes the next qf_prepare extend the pair lists. */
 int qf_set_grad_cutoff(qf_engine *e, double grad_cutoff);
-/* lane = 1 (default): per-lane gradient kernels for the classes with La+Lb+Lc+Ld <= 3,
+/* lane = 1 (default): per-lane gradient kernels for the classes with La+Lb+Lc+Ld <= 5,
  * the wave-per-quartet kernel for the rest; lane = 0: the wave-per-quartet kernel for all */
 int qf_set_grad_kernel(qf_engine *e, int lane);
+/* Per-class gradient kernel (native tuning, no QUICK counterpart; class indices as
+ * qf_set_class_tuning): -1 follows qf_set_grad_kernel, 0 the wave-per-quartet kernel,
+ * 1 the per-lane kernel where the class has one.  With qf_set_timing on, a gradient
+ * call records one event pair per class launch like a Fock build (qf_read_timing). */
+int qf_set_grad_class(qf_engine *e, int cb, int ck, int mode);
 /* device pointers, asynchronous on stream; d_cutmat may be NULL (computed) */
 int qf_eri_grad_device(qf_engine *e, const double *d_P, double *d_grad, const double *d_cutmat, void *stream);
 /* host arrays, synchronous */

@@ -63,6 +63,7 @@ def lib():
         "qf_fock_build_device_noalloc": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "qf_set_grad_cutoff": (ctypes.c_int, [vp, ctypes.c_double]),
         "qf_set_grad_kernel": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_set_grad_class": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "qf_eri_grad_device": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "qf_eri_grad_host": (ctypes.c_int, [vp, dp, dp, dp]),
         "qf_grad_launch_flops": (ctypes.c_int, [vp, dp, ctypes.POINTER(ctypes.c_longlong),

@@ -155,6 +155,28 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "autotune.json")
 
 
+def tune_grad(engine, grad_call, calls=2):
+    """Chooses each class's gradient kernel (per-lane or wave-per-quartet)
+    from event-timed gradient calls of both kinds: ``grad_call()`` runs one
+    gradient on the engine's stream.  Leaves the faster kernel set per class
+    and returns {class name: (mode, ms)}."""
+    names = sorted({nm for nm, _, _ in engine.launches()})
+    times = {}
+    for mode in (1, 0):
+        for nm in names:
+            engine.set_grad_class(*_class_index(nm), mode)
+        grad_call()   # warm-up
+        times[mode] = _class_times(grad_call, engine.launches, engine.read_timing, engine.set_timing, calls)
+    chosen = {}
+    for nm in names:
+        tot = {m: sum(v for (k, _), v in times[m].items() if k == nm) for m in (1, 0)}
+        best = min(tot, key=tot.get)
+        engine.set_grad_class(*_class_index(nm), best)
+        chosen[nm] = (best, tot[best])
+    tune_grad.last_times = times
+    return chosen
+
+
 def size_bucket(nbasis):
     """The engine's own size classes (primitive piece defaults): <=128, <=256, more."""
     return "n<=128" if nbasis <= 128 else "n<=256" if nbasis <= 256 else "n>256"

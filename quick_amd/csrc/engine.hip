@@ -267,6 +267,7 @@ struct qf_engine {
         int tile = -1, waves = 0, tpg = 0, mmax = 0;   // <0 / 0: engine default
         int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
         int os = -1;   // open-shell form: -1 default (2), 0 fused, 1 two passes, 2 fused lean (tile kernels)
+        int grad = -1; // gradient kernel: -1 engine default, 0 wave-per-quartet, 1 per-lane
     };
     ClassTune ctune[kNPC][kNPC];
     // pair blocks
@@ -1373,15 +1374,31 @@ int grad_pass(qf_engine* e, const double* dP, const double* dcm, double* dgrad, 
     qf::MolDesc* dmol = e->d_mol.p + 3;
     QF_HIP(hipMemcpyAsync(dmol, &e->mol[3], sizeof(md), hipMemcpyHostToDevice, s));
     const size_t tile_bytes = (size_t)3 * e->natom * sizeof(double);
-    for (const auto& L : e->launches) {
-        const qf::GradLaunch lane = e->grad_lane ? qf::grad_lane_launcher(L.cb, L.ck) : nullptr;
+    for (size_t li = 0; li < e->launches.size(); ++li) {
+        const auto& L = e->launches[li];
+        const int gm = e->ctune[L.cb][L.ck].grad;
+        const bool want_lane = gm >= 0 ? gm == 1 : e->grad_lane;
+        const qf::GradLaunch lane = want_lane ? qf::grad_lane_launcher(L.cb, L.ck) : nullptr;
         const size_t slots = lane ? 0 : (size_t)qf::grad_lds_doubles(L.cb, L.ck) * sizeof(double);
         const size_t tile = slots + tile_bytes <= kGradLdsMax ? tile_bytes : 0;
         // one workgroup per task unless the engine caps the grid (qf_set_persistent):
         // the hardware then hands out tasks as workgroups finish
         const int grid = e->persist > 0 ? std::min(L.count, e->persist) : L.count;
+        hipEvent_t eb = nullptr;
+        if (e->timing) {
+            while (e->ev_pool.size() < e->ev_used + 2) {
+                hipEvent_t ev;
+                QF_HIP(hipEventCreate(&ev));
+                e->ev_pool.push_back(ev);
+            }
+            eb = e->ev_pool[e->ev_used + 1];
+            QF_HIP(hipEventRecord(e->ev_pool[e->ev_used], s));
+            e->ev_used += 2;
+            e->ev_launch.push_back((int)li);
+        }
         (lane ? lane : qf::grad_launcher(L.cb, L.ck))(e->d_tasks.p + L.offset, L.count, e->d_blocks.p, dmol, grid,
                                                       tile, e->d_gscratch.p, L.fp32, s);
+        if (eb) QF_HIP(hipEventRecord(eb, s));
     }
     QF_HIP(hipGetLastError());
     return QF_OK;
@@ -1401,6 +1418,15 @@ int qf_set_grad_kernel(qf_engine* e, int lane)
 {
     if (!e) return fail(QF_ERR_ARG, "engine is NULL");
     e->grad_lane = lane != 0;
+    return QF_OK;
+}
+
+int qf_set_grad_class(qf_engine* e, int cb, int ck, int mode)
+{
+    if (!e) return fail(QF_ERR_ARG, "engine is NULL");
+    if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (mode < -1 || mode > 1) return fail(QF_ERR_ARG, "mode must be -1, 0 (wave-per-quartet) or 1 (per-lane)");
+    e->ctune[cb][ck].grad = mode;
     return QF_OK;
 }
 

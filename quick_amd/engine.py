@@ -137,6 +137,11 @@ class FockEngine:
         wave-per-quartet kernel for every class."""
         L.check(self.lib.qf_set_grad_kernel(self.h, 1 if lane else 0), "qf_set_grad_kernel")
 
+    def set_grad_class(self, cb, ck, mode=-1):
+        """Gradient kernel of class (cb|ck): -1 as set_grad_kernel, 0 the
+        wave-per-quartet kernel, 1 the per-lane kernel where the class has one."""
+        L.check(self.lib.qf_set_grad_class(self.h, int(cb), int(ck), int(mode)), "qf_set_grad_class")
+
     def eri_grad(self, P, cutmatrix=None):
         """d/dR of 1/2 sum P G(P) with P fixed, (natom, 3) in Eh/bohr (QUICK's
         gpu_get_cshell_eri_grad_); synchronous."""

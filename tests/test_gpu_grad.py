@@ -47,6 +47,30 @@ def test_eri_gradient_matches_oracle(name, basis, cutoff, lane):
     assert np.max(np.abs(g.sum(axis=0))) <= 1e-11 * max(1.0, float(np.max(np.abs(g))))
 
 
+def test_eri_gradient_per_class_kernels_and_timing():
+    """qf_set_grad_class: a mix of per-lane and wave-per-quartet classes gives
+    the oracle's gradient, and a timed gradient call records every class
+    launch (the event pairs qf_read_timing reads)."""
+    from quick_amd.autotune import _class_index
+    announce("ERI gradient: gly1 6-31G* per-class kernel mix, timed")
+    mol = molecule("gly1")
+    orc = oracle_for(mol, "6-31G*")
+    P, _ = seeded_density_for(orc, mol)
+    g_ref, _ = orc.eri_grad(P, grad_cutoff=1e-11, prim_limit=1e-12, threads=8)
+    eng = _engine(product_basis(mol, "6-31G*"), cutoff=1e-11)
+    names = sorted({nm for nm, _, _ in eng.launches()})
+    for k, nm in enumerate(names):   # alternate the kernels over the classes
+        eng.set_grad_class(*_class_index(nm), k % 2)
+    eng.set_timing(True)
+    g = eng.eri_grad(P)
+    ms, calls = eng.read_timing()
+    eng.set_timing(False)
+    assert _err(g, g_ref) <= GRAD_TOL
+    assert len(ms) == len(eng.launches()) and np.all(calls == 1) and np.all(ms >= 0.0) and ms.sum() > 0.0
+    with pytest.raises(RuntimeError):
+        eng.set_grad_class(0, 1, 0)   # ck > cb
+
+
 def test_eri_gradient_quick_cutoffs_and_lists():
     """QUICK's production settings (integral cutoff 1e-9, gradCutoff 1e-7,
     primLimit 1e-10) and a gradCutoff below the integral cutoff, which
