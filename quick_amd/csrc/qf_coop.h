@@ -407,7 +407,7 @@ __global__ __launch_bounds__(D::NT) void fock_coop_kernel(const Task* __restrict
                 mask &= mask - 1;
                 const int j = t.ket_start + v;
                 const PairInfo ki = cptr(kb.info)[j];
-                const double dnv = __shfl(dn, v, WAVE);
+                const double dnv = wave_read_lane(dn, v);
                 coop_quartet<D, OS>(t, bb, i, bi, kb, j, ki, dnv, md, G, G2, s, QF_BOYS_LDS_PTR);
             }
         }

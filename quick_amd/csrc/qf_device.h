@@ -55,6 +55,15 @@ __host__ __device__ __forceinline__ int wave_uniform(int x)
 #endif
 }
 
+// x of lane `src` (a wave-uniform index) in every lane, as a scalar value
+__device__ __forceinline__ double wave_read_lane(double x, int src)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // 1/sqrt(x): the hardware reciprocal square root on the device, the C
 // library on the host (the host path only serves CPU checks of this code).
 __host__ __device__ __forceinline__ double qf_rsqrt(double x)
