@@ -530,6 +530,18 @@ MIN_WAVES_OVERRIDE = {(2, 0, 1, 0): 2, (2, 0, 2, 0): 1, (1, 1, 0, 0): 4, (0, 0, 
 # density blocks: one wave fewer than the closed-shell kernel by default
 MIN_WAVES_OS_OVERRIDE = {(1, 0, 0, 0): 3}
 
+# Classes whose FP64 tile kernels take the first exchange lean (LEAN_K: the
+# a-row density blocks re-read and the a-row exchange terms added to the LDS
+# tile every quartet instead of held in registers over the task).  Chosen
+# from per-class times of both forms with the persisted launch shapes, ms
+# per build (H2O)32 6-31G(d) / (Gly)10 6-31G(d,p): (ps|ps) 0.574 -> 0.538 /
+# 1.43 -> 1.36, (pp|ss) 0.321 -> 0.307 / 0.773 -> 0.753, (pp|pp) 0.206 ->
+# 0.198 / 0.83 -> 0.81, (ds|ss) 0.550 -> 0.547 / 0.697 -> 0.648, (dp|ss)
+# 0.321 -> 0.281 / 0.655 -> 0.630, (dp|ds) 0.647 -> 0.557 / 2.00 -> 1.77,
+# (dd|ps) 0.243 -> 0.210 / 0.70 -> 0.57.  Slower in that form: (ss|ss) 1.10
+# -> 1.14, (ps|ss) 1.29 -> 1.33, (ds|ps) 0.54 -> 0.58, (dp|pp) 0.73 -> 0.90.
+LEAN_K_CLASSES = {(1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 1), (2, 0, 0, 0), (2, 1, 0, 0), (2, 1, 2, 0), (2, 2, 1, 0)}
+
 
 def min_waves(nacc, ni):
     """Occupancy target per class (caps VGPRs: 4 waves -> 128, 3 -> 168, 2 -> 256).
@@ -584,6 +596,8 @@ def gen_class(La, Lb, Lc, Ld):
     src.append("  static constexpr int MIN_WAVES = %d;   // waves per SIMD the register budget targets" % mw)
     src.append("  static constexpr int MIN_WAVES_OS = %d;   // the same for the open-shell kernels"
                % MIN_WAVES_OS_OVERRIDE.get((La, Lb, Lc, Ld), max(1, mw - 1)))
+    src.append("  static constexpr bool LEAN_K = %s;   // FP64 tile kernels: first exchange re-read per quartet"
+               % ("true" if (La, Lb, Lc, Ld) in LEAN_K_CLASSES else "false"))
     src.append("  template <typename R>")
     src.append("  __host__ __device__ static __forceinline__ void vrr(const R (&F)[L + 1], const VrrPar<R>& p, R (&acc)[NACC]) {")
     src += ["    " + l for l in vlines]

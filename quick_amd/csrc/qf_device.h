@@ -489,6 +489,8 @@ struct TileRef {
 // (tile kernels): the second exchange keeps no per-task state in registers;
 // its a-row density blocks are re-read and its a-row terms added to the tile
 // every quartet, so the contraction runs at the closed-shell register count.
+// Classes with C::LEAN_K treat the first exchange the same way in their FP64
+// tile kernels (the a-row blocks of P and terms of K_ac, K_ad).
 template <class C, typename R, bool TILE, bool OS, bool LEAN = false>
 __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, const PairBlock& kb,
                                           const MolDesc& md, QF_AS_GLOBAL double* G, QF_AS_GLOBAL double* G2,
@@ -517,9 +519,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 
     const auto P2 = gptr(md.P2);
     constexpr bool kLean = OS && TILE && LEAN;
+    // the same for the first exchange, per class (C::LEAN_K, FP64 tile kernels)
+    constexpr bool kLeanK = TILE && C::LEAN_K && !kFP32;
+    constexpr int t1 = kLeanK ? 0 : 1;
     constexpr int os2 = (OS && !kLean) ? 1 : 0;   // sizes of the per-task second-density arrays (1 when unused)
     constexpr int osq = OS ? 1 : 0;               // the same for the per-quartet arrays
-    double Dcd[nc * nd], Dac[na * nc], Dad[na * nd];
+    double Dcd[nc * nd], Dac[t1 * na * nc + 1 - t1], Dad[t1 * na * nd + 1 - t1];
     double Dac2[os2 * na * nc + 1 - os2], Dad2[os2 * na * nd + 1 - os2];
 #pragma unroll
     for (int c = 0; c < nc; ++c)
@@ -527,10 +532,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         for (int d = 0; d < nd; ++d) Dcd[c * nd + d] = PJ[(ki.fa + c) * n + ki.fb + d];
 #pragma unroll
     for (int a = 0; a < na; ++a) {
+        if constexpr (!kLeanK) {
 #pragma unroll
-        for (int c = 0; c < nc; ++c) Dac[a * nc + c] = P[(fa + a) * n + ki.fa + c];
+            for (int c = 0; c < nc; ++c) Dac[a * nc + c] = P[(fa + a) * n + ki.fa + c];
 #pragma unroll
-        for (int d = 0; d < nd; ++d) Dad[a * nd + d] = P[(fa + a) * n + ki.fb + d];
+            for (int d = 0; d < nd; ++d) Dad[a * nd + d] = P[(fa + a) * n + ki.fb + d];
+        }
         if constexpr (OS && !kLean) {
 #pragma unroll
             for (int c = 0; c < nc; ++c) Dac2[a * nc + c] = P2[(fa + a) * n + ki.fa + c];
@@ -538,14 +545,16 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
             for (int d = 0; d < nd; ++d) Dad2[a * nd + d] = P2[(fa + a) * n + ki.fb + d];
         }
     }
-    double Jcd[nc * nd], Kac[na * nc], Kad[na * nd];
+    double Jcd[nc * nd], Kac[t1 * na * nc + 1 - t1], Kad[t1 * na * nd + 1 - t1];
     double Kac2[os2 * na * nc + 1 - os2], Kad2[os2 * na * nd + 1 - os2];
 #pragma unroll
     for (int k = 0; k < nc * nd; ++k) Jcd[k] = 0.0;
+    if constexpr (!kLeanK) {
 #pragma unroll
-    for (int k = 0; k < na * nc; ++k) Kac[k] = 0.0;
+        for (int k = 0; k < na * nc; ++k) Kac[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < na * nd; ++k) Kad[k] = 0.0;
+        for (int k = 0; k < na * nd; ++k) Kad[k] = 0.0;
+    }
     if constexpr (OS && !kLean) {
 #pragma unroll
         for (int k = 0; k < na * nc; ++k) Kac2[k] = 0.0;
@@ -602,6 +611,24 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         double Kbc2[osq * nb * nc + 1 - osq], Kbd2[osq * nb * nd + 1 - osq];
         constexpr int lq = kLean ? 1 : 0;
         double Dac2q[lq * na * nc + 1 - lq], Dad2q[lq * na * nd + 1 - lq];
+        constexpr int lk = kLeanK ? 1 : 0;
+        double Dacq[lk * na * nc + 1 - lk], Dadq[lk * na * nd + 1 - lk];
+        double Kacq[lk * na * nc + 1 - lk], Kadq[lk * na * nd + 1 - lk];
+        if constexpr (kLeanK) {
+            int oc = fa * n + ki.fa, od = fa * n + ki.fb;
+            asm volatile("" : "+v"(oc), "+v"(od));
+#pragma unroll
+            for (int a = 0; a < na; ++a) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) Dacq[a * nc + c] = P[oc + a * n + c];
+#pragma unroll
+                for (int d = 0; d < nd; ++d) Dadq[a * nd + d] = P[od + a * n + d];
+            }
+#pragma unroll
+            for (int k = 0; k < na * nc; ++k) Kacq[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < na * nd; ++k) Kadq[k] = 0.0;
+        }
         double Kac2q[lq * na * nc + 1 - lq], Kad2q[lq * na * nd + 1 - lq];
 #pragma unroll
         for (int k = 0; k < na * nb; ++k) Jab[k] = 0.0;
@@ -651,10 +678,17 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         const double v = double(I[((a * nb + b) * nc + c) * nd + d]) * (cons * s);
                         Jab[a * nb + b] += Dcd[c * nd + d] * v;
                         Jcd[c * nd + d] += Dab[a * nb + b] * v;
-                        Kac[a * nc + c] += Dbd[b * nd + d] * v;
-                        Kad[a * nd + d] += Dbc[b * nc + c] * v;
-                        Kbc[b * nc + c] += Dad[a * nd + d] * v;
-                        Kbd[b * nd + d] += Dac[a * nc + c] * v;
+                        if constexpr (kLeanK) {
+                            Kacq[a * nc + c] += Dbd[b * nd + d] * v;
+                            Kadq[a * nd + d] += Dbc[b * nc + c] * v;
+                            Kbc[b * nc + c] += Dadq[a * nd + d] * v;
+                            Kbd[b * nd + d] += Dacq[a * nc + c] * v;
+                        } else {
+                            Kac[a * nc + c] += Dbd[b * nd + d] * v;
+                            Kad[a * nd + d] += Dbc[b * nc + c] * v;
+                            Kbc[b * nc + c] += Dad[a * nd + d] * v;
+                            Kbd[b * nd + d] += Dac[a * nc + c] * v;
+                        }
                         if constexpr (kLean) {
                             Kac2q[a * nc + c] += Dbd2[b * nd + d] * v;
                             Kad2q[a * nd + d] += Dbc2[b * nc + c] * v;
@@ -707,6 +741,16 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
                         }
                     }
                 }
+                if constexpr (kLeanK) {
+#pragma unroll
+                    for (int a = 0; a < na; ++a) {
+                        QF_AS_LDS double* row = tr.k + a * tr.ncol;
+#pragma unroll
+                        for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kacq[a * nc + c]);
+#pragma unroll
+                        for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kadq[a * nd + d]);
+                    }
+                }
                 if constexpr (kLean) {
 #pragma unroll
                     for (int a = 0; a < na; ++a) {
@@ -756,10 +800,12 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
 #pragma unroll
         for (int a = 0; a < na; ++a) {
             QF_AS_LDS double* row = tr.k + a * tr.ncol;
+            if constexpr (!kLeanK) {
 #pragma unroll
-            for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kac[a * nc + c]);
+                for (int c = 0; c < nc; ++c) lds_add(&row[ki.cola + c], Kac[a * nc + c]);
 #pragma unroll
-            for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kad[a * nd + d]);
+                for (int d = 0; d < nd; ++d) lds_add(&row[ki.colb + d], Kad[a * nd + d]);
+            }
             if constexpr (OS && !kLean) {
                 if (tr.k2) {
                     QF_AS_LDS double* row2 = tr.k2 + a * tr.ncol;
