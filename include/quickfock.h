@@ -87,6 +87,10 @@ int qf_set_precision(qf_engine *e, int mode, double split);
  * default): 0 (all of them) above 256 basis functions, 5 up to 256.  The
  * others run every quartet in FP64. */
 int qf_set_split_classes(qf_engine *e, int min_lsum);
+/* Per-class override of that choice (native tuning, no QUICK counterpart; class
+ * indices as qf_set_class_tuning): -1 follows qf_set_split_classes, 0 the class
+ * runs every quartet in FP64, 1 it is split (where it has per-lane kernels). */
+int qf_set_class_split(qf_engine *e, int cb, int ck, int mode);
 
 /* Schwarz bounds.  Pass NULL to have them computed on the GPU, or QUICK's
  * Ycutoff[nshell*nshell] and cutprim[nprim_total*nprim_total]. */

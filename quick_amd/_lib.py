@@ -55,6 +55,7 @@ def lib():
         "qf_set_cutoffs": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "qf_set_precision": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
         "qf_set_split_classes": (ctypes.c_int, [vp, ctypes.c_int]),
+        "qf_set_class_split": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "qf_set_schwarz": (ctypes.c_int, [vp, dp, dp]),
         "qf_get_schwarz": (ctypes.c_int, [vp, dp, dp]),
         "qf_prepare": (ctypes.c_int, [vp]),

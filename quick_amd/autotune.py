@@ -53,7 +53,7 @@ def _class_index(name):
 
 
 def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, make_density=None, densities=None,
-             pieces=PIECES, build=None, os_modes=None):
+             pieces=PIECES, build=None, os_modes=None, split_modes=None):
     """Choose a launch shape per class, then a primitive piece size per class.
     ``engines``: one FockEngine or a list (tuned as one FockBatch).  Single
     engine: ``make_density(0)`` returns the device pointer (int) of the
@@ -63,7 +63,10 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
     (piece,)) and leaves the best settings applied.  ``os_modes`` (e.g.
     (0, 1, 2) with an open-shell ``build``): a third pass choosing each
     class's open-shell form (fused, two closed-shell passes, fused lean),
-    appended to the candidate."""
+    appended to the candidate.  ``split_modes`` (e.g. (0, 1) for mixed
+    engines): a last pass choosing per class between FP64 only and the
+    precision split, timed over both halves; the candidate then carries
+    (..., piece, open-shell form or -1, split mode)."""
     from .engine import FockBatch
     single = not isinstance(engines, (list, tuple))
     engs = [engines] if single else list(engines)
@@ -144,6 +147,27 @@ def autotune(engines, stream, builds=2, candidates=CANDIDATES, batch_kw=None, ma
             cand, _ = chosen[name]
             chosen[name] = (tuple(cand) + (best,), ototal(best))
         autotune.last_os_times = otimes
+    if split_modes:
+        stimes = {}
+        for sm in split_modes:
+            for e in engs:
+                for cb in range(NPC):
+                    for ck in range(cb + 1):
+                        e.set_class_split(cb, ck, sm)
+            stimes[sm] = timed()
+        for name in names:
+            cb, ck = _class_index(name)
+
+            def stotal(sm):
+                keys = [k for k in stimes[sm] if k[0] == name]
+                return sum(stimes[sm][k] for k in keys) if keys else float("inf")
+            best = min(split_modes, key=stotal)
+            for e in engs:
+                e.set_class_split(cb, ck, best)
+            cand, _ = chosen[name]
+            cand = tuple(cand) + ((-1,) if len(cand) < 6 else ())
+            chosen[name] = (cand + (best,), stotal(best))
+        autotune.last_split_times = stimes
     return chosen
 
 
@@ -199,7 +223,7 @@ def table_key(engines, batched=False, device_name=None, tag=None):
 
 
 def load_table(key, path=TABLE):
-    """{class name: (digestion, waves, tasks/group, bras/task, piece[, open-shell form])} or None."""
+    """{class name: (digestion, waves, tasks/group, bras/task, piece[, open-shell form[, split mode]])} or None."""
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -229,6 +253,8 @@ def apply_table(engines, entry):
             e.set_class_piece(cb, ck, cand[4])
             if len(cand) > 5:
                 e.set_class_os(cb, ck, cand[5])
+            if len(cand) > 6:
+                e.set_class_split(cb, ck, cand[6])
 
 
 def tune_or_load(engines, stream, batched=False, retune=False, path=TABLE, save_path=None, tag=None, **kw):

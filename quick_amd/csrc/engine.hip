@@ -268,6 +268,7 @@ struct qf_engine {
         int piece = 0;                                  // primitive pairs per piece: 0 engine default, < 0 none
         int os = -1;   // open-shell form: -1 default (2), 0 fused, 1 two passes, 2 fused lean (tile kernels)
         int grad = -1; // gradient kernel: -1 engine default, 0 wave-per-quartet, 1 per-lane
+        int split = -1; // mixed builds: -1 engine rule (split_lmin), 0 all FP64, 1 split
     };
     ClassTune ctune[kNPC][kNPC];
     // pair blocks
@@ -730,7 +731,8 @@ int prepare(qf_engine* e)
                 // to pay (256 batched molecules: 1389 builds/s splitting all,
                 // FP64 1432).
                 const int lmin = e->split_lmin >= 0 ? e->split_lmin : e->nbasis <= 256 ? 5 : 0;
-                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class && Lsum_s >= lmin;
+                const bool split_class = e->prec == QF_PREC_MIXED && mfit >= 1 && lane_class &&
+                                         (ct.split >= 0 ? ct.split == 1 : Lsum_s >= lmin);
                 if (fp32 && !split_class) continue;
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
@@ -1259,6 +1261,16 @@ int qf_set_split_classes(qf_engine* e, int min_lsum)
 {
     if (!e || min_lsum < -1) return fail(QF_ERR_ARG, "min_lsum must be >= -1");
     e->split_lmin = min_lsum;
+    e->prepared = false;
+    return QF_OK;
+}
+
+int qf_set_class_split(qf_engine* e, int cb, int ck, int mode)
+{
+    if (!e) return fail(QF_ERR_ARG, "NULL");
+    if (cb < 0 || cb >= kNPC || ck < 0 || ck > cb) return fail(QF_ERR_ARG, "class index out of range");
+    if (mode < -1 || mode > 1) return fail(QF_ERR_ARG, "mode must be -1, 0 (FP64) or 1 (split)");
+    e->ctune[cb][ck].split = mode;
     e->prepared = false;
     return QF_OK;
 }

@@ -77,6 +77,10 @@ class FockEngine:
         """Mixed builds split the classes with La+Lb+Lc+Ld >= min_lsum (-1: by molecule size)."""
         L.check(self.lib.qf_set_split_classes(self.h, int(min_lsum)), "qf_set_split_classes")
 
+    def set_class_split(self, cb, ck, mode=-1):
+        """Mixed builds, class (cb|ck): -1 as set_split_classes, 0 all FP64, 1 split."""
+        L.check(self.lib.qf_set_class_split(self.h, int(cb), int(ck), int(mode)), "qf_set_class_split")
+
     def schwarz(self):
         ns = len(self.basis.shells)
         y = np.zeros((ns, ns))

@@ -43,6 +43,36 @@ def test_polyglycine_mixed_fock_vs_oracle(n):
     assert dF <= 1e-6 and dE <= 1e-6
 
 
+def test_per_class_split_choice():
+    """qf_set_class_split: with every class forced to FP64 a mixed engine
+    builds the FP64 G exactly (no FP32 tasks); with the split alternating over
+    the classes it stays within the mixed bar of the oracle."""
+    from quick_amd.autotune import _class_index
+    announce("config3 (Gly)2 6-31G** per-class split choice")
+    mol = molecule("gly2")
+    orc = oracle_for(mol, BASIS)
+    P, _ = seeded_density_for(orc, mol)
+    G_ref, _ = orc.fock2e(P, cutoff=1e-11, threads=8)
+    bs = product_basis(mol, BASIS)
+    G64 = _engine(bs, cutoff=1e-11).fock2e(P)
+    emx = _engine(bs, cutoff=1e-11, precision="mixed", split=1e-4)
+    emx.set_split_classes(0)
+    names = sorted({nm for nm, _, _ in emx.launches()})
+    for nm in names:
+        emx.set_class_split(*_class_index(nm), 0)
+    G0 = emx.fock2e(P)
+    assert emx.stats()["n_tasks_fp32"] == 0
+    assert float(np.max(np.abs(G0 - G64))) <= 1e-13
+    for k, nm in enumerate(names):
+        emx.set_class_split(*_class_index(nm), k % 2)
+    Gm = emx.fock2e(P)
+    st = emx.stats()
+    dF = float(np.max(np.abs(Gm - G_ref)))
+    print(f"  FP32 tasks {st['n_tasks_fp32']}, max|dF| {dF:.2e} Eh", flush=True)
+    assert st["n_tasks_fp32"] > 0 and dF <= 1e-6
+    assert {f for _, f, _ in emx.launches()} == {False, True}
+
+
 def test_polyglycine_mixed_scf_vs_oracle_scf():
     """H-(Gly)2-OH: SCF with mixed-precision GPU builds against the oracle's
     FP64 SCF, |dE_SCF| <= 1e-6 Eh (and the FP64 GPU SCF to 1e-8)."""
