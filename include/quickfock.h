@@ -257,8 +257,9 @@ int qf_set_persistent(qf_engine *e, int max_workgroups);
 /* Primitive pieces (no QUICK counterpart: a launch-shape choice).  Each task
  * sums at most `piece` bra and `piece` ket primitive pairs of its quartets, so
  * the most contracted quartets of a small molecule are spread over several
- * waves; the pieces add up to the same Fock matrix.  piece = 0: automatic (6
- * up to 256 basis functions, 12 above); piece < 0: never split. */
+ * waves; the pieces add up to the same Fock matrix.  piece = 0: automatic (1
+ * up to 96 basis functions, 2 up to 128, 4 up to 256, 12 above); piece < 0:
+ * never split. */
 int qf_set_prim_split(qf_engine *e, int piece);
 /* Per-class primitive piece size (class indices as qf_set_class_tuning):
  * 0 follows qf_set_prim_split, < 0 never splits the class. */

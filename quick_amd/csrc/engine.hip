@@ -679,13 +679,18 @@ int prepare(qf_engine* e)
     // Primitive pieces: a wave's lanes each run the whole primitive loop of
     // one quartet, so the most contracted quartets set the tail of a launch
     // (and, for small molecules with few waves, its whole time).  The
-    // primitive pairs of a task are cut into pieces of at most 2 up to 128
-    // functions, 4 up to 256 and 12 above (e.g. 36 -> 3 x 12).  Measured on
-    // (H2O)32 6-31G(d): no split 17.0 ms, pieces of 6 / 12 / 18: 15.8 / 15.4 /
-    // 15.9 ms; on QUICK's regression inputs (13-210 functions) pieces of 2-4
-    // beat 6.
+    // primitive pairs of a task are cut into pieces of at most 1 up to 96
+    // functions, 2 up to 128, 4 up to 256 and 12 above (e.g. 36 -> 3 x 12).
+    // Measured on (H2O)32 6-31G(d): no split 17.0 ms, pieces of 6 / 12 / 18:
+    // 15.8 / 15.4 / 15.9 ms; on QUICK's 69 regression inputs (12-210
+    // functions), geometric mean of the build times up to 32 functions with
+    // pieces of 1 / 2: 0.39 / 0.55 ms, 33-64: 0.93 / 1.23 ms, 79 functions
+    // 2.20 / 2.49 ms, 114: 2.02 / 1.50 ms, 210 (pieces of 1 / 4): 5.53 /
+    // 4.57 ms.  A small molecule has too few quartets to fill the GPU, and
+    // one primitive quartet per lane gives it the most waves.
     const int piece = e->prim_piece > 0   ? e->prim_piece
                       : e->prim_piece < 0 ? 1 << 20
+                      : e->nbasis <= 96   ? 1
                       : e->nbasis <= 128  ? 2
                       : e->nbasis <= 256  ? 4
                                           : 12;
