@@ -234,7 +234,8 @@ int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
  * enable = 1 accumulates J_ab and all exchange terms of a bra chunk in a
  * workgroup-shared tile and adds it to G once, instead of one FP64 atomic
  * per quartet.  waves: waves per workgroup (1..4); tasks_per_group: tasks
- * one workgroup takes; lds_bytes: tile budget (1 KiB..64 KiB).  Classes
+ * one workgroup takes (0, the default: 4 up to 256 basis functions, 16
+ * above); lds_bytes: tile budget (1 KiB..64 KiB).  Classes
  * whose tile does not fit fall back to direct atomics.  With enable = 1 the
  * d-heavy classes (La+Lb+Lc+Ld >= 6, or >= 4 up to 128 basis functions)
  * default to the wave-per-quartet kernel;

@@ -231,7 +231,8 @@ class FockEngine:
     def set_tuning(self, replicas=8, shuffle=True):
         L.check(self.lib.qf_set_tuning(self.h, int(replicas), 1 if shuffle else 0), "qf_set_tuning")
 
-    def set_tile(self, enable=True, waves=4, tasks_per_group=16, lds_bytes=64 * 1024):
+    def set_tile(self, enable=True, waves=4, tasks_per_group=0, lds_bytes=64 * 1024):
+        """LDS-tile digestion on/off; tasks_per_group 0: 4 up to 256 functions, 16 above."""
         L.check(self.lib.qf_set_tile(self.h, 1 if enable else 0, int(waves), int(tasks_per_group), int(lds_bytes)),
                 "qf_set_tile")
 
