@@ -237,7 +237,7 @@ int qf_set_tuning(qf_engine *e, int replicas, int shuffle);
  * one workgroup takes (0, the default: 4 up to 256 basis functions, 16
  * above); lds_bytes: tile budget (1 KiB..64 KiB).  Classes
  * whose tile does not fit fall back to direct atomics.  With enable = 1 the
- * d-heavy classes (La+Lb+Lc+Ld >= 6, or >= 4 up to 128 basis functions)
+ * d-heavy classes (La+Lb+Lc+Ld >= 6; >= 5 up to 128 basis functions, >= 4 up to 48)
  * default to the wave-per-quartet kernel;
  * enable = 0 runs every class without a qf_set_class_tuning entry on the
  * per-lane direct-atomic kernels. */

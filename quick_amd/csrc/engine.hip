@@ -755,14 +755,18 @@ int prepare(qf_engine* e)
                 // wave-per-quartet kernel (FP64 launches of classes that have one)
                 // wave-per-quartet kernel: where the class tuning asks for it, and by
                 // default (unless qf_set_tile turned the shared-memory kernels off)
-                // for the d-heavy classes (La+Lb+Lc+Ld >= 6; >= 4 up to 128 functions,
+                // for the d-heavy classes (La+Lb+Lc+Ld >= 6; >= 5 up to 128 functions and >= 4 up to 48,
                 // where few waves run and latency decides), whose per-lane
                 // kernels spill several KB per lane: one such quartet alone on a
                 // lane costs milliseconds, which dominates the build of a small
                 // molecule
                 const int Lsum = kClassL[cb][0] + kClassL[cb][1] + kClassL[ck][0] + kClassL[ck][1];
-                const bool want_coop = !lane_class || ct.tile == 2 ||
-                                       (ct.tile < 0 && e->tile != 0 && Lsum >= (e->nbasis <= 128 ? 4 : 6));
+                // (QUICK's regression inputs of 49-128 functions run their
+                // La+Lb+Lc+Ld = 4 classes faster on the tile kernel: PO4 6-31G*
+                // 15.8x -> 18.0x, PSB5 6-31G 26x -> 34x QUICK's build times;
+                // up to 48 functions and at 5 the wave-per-quartet kernel wins)
+                const int coop_lmin = e->nbasis <= 48 ? 4 : e->nbasis <= 128 ? 5 : 6;
+                const bool want_coop = !lane_class || ct.tile == 2 || (ct.tile < 0 && e->tile != 0 && Lsum >= coop_lmin);
                 const bool coop = !fp32 && want_coop && qf::fock_coop_launcher(cb, ck) != nullptr;
                 if (!coop && !lane_class) return fail(QF_ERR_STATE, "class has neither per-lane nor wave-per-quartet kernels");
                 const bool tile = !coop && (fp32 || (ct.tile >= 0 ? ct.tile != 0 : e->tile != 0)) && mfit >= 1;
