@@ -615,6 +615,8 @@ __device__ __forceinline__ void fock_task(const Task& t, const PairBlock& bb, co
         double Dacq[lk * na * nc + 1 - lk], Dadq[lk * na * nd + 1 - lk];
         double Kacq[lk * na * nc + 1 - lk], Kadq[lk * na * nd + 1 - lk];
         if constexpr (kLeanK) {
+            // the a-row blocks of P again for this quartet, from offsets the
+            // compiler cannot hoist out of the bra loop (see the P2 blocks below)
             int oc = fa * n + ki.fa, od = fa * n + ki.fb;
             asm volatile("" : "+v"(oc), "+v"(od));
 #pragma unroll
