@@ -122,3 +122,88 @@ def test_autotune_table_round_trip(tmp_path):
     assert A.load_table("dev|batch|mixed|n<=128", path) == {"(ps|ss)": (0, 0, 0, 2, -1)}
     assert A.load_table("missing", path) is None
     assert A.size_bucket(128) == "n<=128" and A.size_bucket(200) == "n<=256" and A.size_bucket(608) == "n>256"
+
+
+class _MockEngine:
+    """Stands in for a FockEngine in the autotuner: per-class launch times
+    follow a fixed cost table of the current settings (no GPU)."""
+    NAMES = {(0, 0): "(ss|ss)", (1, 0): "(ps|ss)"}
+
+    def __init__(self):
+        self.tune = {k: (-1, 0, 0, 0) for k in self.NAMES}
+        self.piece = {k: 0 for k in self.NAMES}
+        self.os = {k: -1 for k in self.NAMES}
+        self.split = {k: -1 for k in self.NAMES}
+        self.acc = None
+
+    def set_class_tuning(self, cb, ck, tile=-1, waves=0, tpg=0, mmax=0):
+        if (cb, ck) in self.NAMES:
+            self.tune[(cb, ck)] = (tile, waves, tpg, mmax)
+
+    def set_class_piece(self, cb, ck, piece):
+        if (cb, ck) in self.NAMES:
+            self.piece[(cb, ck)] = piece
+
+    def set_class_os(self, cb, ck, mode):
+        if (cb, ck) in self.NAMES:
+            self.os[(cb, ck)] = mode
+
+    def set_class_split(self, cb, ck, mode):
+        if (cb, ck) in self.NAMES:
+            self.split[(cb, ck)] = mode
+
+    def launches(self):
+        out = []
+        for k, nm in self.NAMES.items():
+            out.append((nm, False, 1))
+            if self.split[k] == 1:
+                out.append((nm, True, 1))
+        return out
+
+    def cost(self, k, fp32):
+        # (ss|ss) is fastest on the tile kernel with pieces of 8, fused lean
+        # and unsplit; (ps|ss) on the wave-per-quartet kernel, unsplit pieces,
+        # two passes and split (its two halves cost less than one FP64 launch)
+        best_tile, best_piece, best_os, split_gain = {(0, 0): (1, 8, 2, -0.1), (1, 0): (2, -1, 1, 0.3)}[k]
+        t = 1.0 + (0.0 if self.tune[k][0] == best_tile else 0.5) + (0.0 if self.piece[k] == best_piece else 0.2)
+        t += 0.0 if self.os[k] in (-1, best_os) else 0.1
+        if self.split[k] == 1:
+            return 0.5 * (t - split_gain)   # each half
+        return t
+
+    def set_timing(self, on=True):
+        self.acc = [[0.0, 0] for _ in self.launches()] if on else None
+
+    def build(self):
+        if self.acc is not None:
+            for i, (nm, f32, _) in enumerate(self.launches()):
+                k = next(k for k, v in self.NAMES.items() if v == nm)
+                self.acc[i][0] += self.cost(k, f32)
+                self.acc[i][1] += 1
+
+    def read_timing(self):
+        import numpy as np
+        return np.array([a[0] for a in self.acc]), np.array([a[1] for a in self.acc])
+
+
+def test_autotune_passes_choose_per_class_and_table_applies_them(tmp_path):
+    """autotune(): the shape, piece, open-shell and split passes each keep the
+    fastest setting per class (split timed over both precision halves); the
+    candidate tuple carries them in that order and apply_table sets them."""
+    from quick_amd import autotune as A
+    e = _MockEngine()
+    cands = [(1, 4, 8, 0), (2, 0, 0, 1)]
+    chosen = A.autotune(e, 0, builds=2, candidates=cands, pieces=[0, 8, -1], build=e.build,
+                        os_modes=(0, 1, 2), split_modes=(0, 1))
+    assert chosen["(ss|ss)"][0] == (1, 4, 8, 0, 8, 2, 0)
+    assert chosen["(ps|ss)"][0] == (2, 0, 0, 1, -1, 1, 1)
+    path = str(tmp_path / "tab.json")
+    A.save_table("k", chosen, path)
+    f = _MockEngine()
+    A.apply_table(f, A.load_table("k", path))
+    assert f.tune[(0, 0)] == (1, 4, 8, 0) and f.piece[(0, 0)] == 8 and f.os[(0, 0)] == 2 and f.split[(0, 0)] == 0
+    assert f.tune[(1, 0)] == (2, 0, 0, 1) and f.piece[(1, 0)] == -1 and f.os[(1, 0)] == 1 and f.split[(1, 0)] == 1
+    # without an open-shell pass the split mode follows a -1 placeholder
+    g = _MockEngine()
+    chosen = A.autotune(g, 0, builds=1, candidates=cands, pieces=[0, 8, -1], build=g.build, split_modes=(0, 1))
+    assert chosen["(ps|ss)"][0] == (2, 0, 0, 1, -1, -1, 1)
