@@ -207,3 +207,26 @@ def test_autotune_passes_choose_per_class_and_table_applies_them(tmp_path):
     g = _MockEngine()
     chosen = A.autotune(g, 0, builds=1, candidates=cands, pieces=[0, 8, -1], build=g.build, split_modes=(0, 1))
     assert chosen["(ps|ss)"][0] == (2, 0, 0, 1, -1, -1, 1)
+
+
+def test_tune_grad_keeps_the_faster_kernel_per_class():
+    """tune_grad(): per class, the gradient kernel (per-lane 1, wave-per-quartet
+    0) with the smaller event-timed sum is left set."""
+    from quick_amd import autotune as A
+
+    class G(_MockEngine):
+        def __init__(self):
+            super().__init__()
+            self.grad = {k: -1 for k in self.NAMES}
+
+        def set_grad_class(self, cb, ck, mode):
+            self.grad[(cb, ck)] = mode
+
+        def cost(self, k, fp32):
+            # (ss|ss) faster per lane, (ps|ss) on the wave-per-quartet kernel
+            return {(0, 0): {1: 1.0, 0: 3.0}, (1, 0): {1: 2.0, 0: 1.5}}[k][self.grad[k]]
+
+    e = G()
+    chosen = A.tune_grad(e, e.build, calls=2)
+    assert chosen["(ss|ss)"][0] == 1 and chosen["(ps|ss)"][0] == 0
+    assert e.grad[(0, 0)] == 1 and e.grad[(1, 0)] == 0
