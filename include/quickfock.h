@@ -248,7 +248,8 @@ int qf_set_tile(qf_engine *e, int enable, int waves, int tasks_per_group, int ld
  * with ck <= cb: tile (-1 engine default, 0 direct atomics, 1 LDS tile, 2
  * one quartet per wave with the recursions in LDS - d-heavy classes; others
  * use the tile), waves per tile workgroup, tasks per tile workgroup (with
- * tile = 2: kets per task, 1..64, default 8) and bras per task (0: default). */
+ * tile = 2: kets per task, 1..64, default 1 up to 256 basis functions and 8
+ * above) and bras per task (0: default). */
 int qf_set_class_tuning(qf_engine *e, int cb, int ck, int tile, int waves, int tasks_per_group,
                         int bras_per_task);
 
